@@ -1,0 +1,202 @@
+!
+! Test-infrastructure driver for the reference BOXZYHYDRO hydro step.
+!
+! This program is part of the oracle: it links the reference's own Fortran
+! subroutines (state, velocity, courant, source, flux, set_ghost_cells, avisc,
+! init_grid, the eos and units setup) compiled from their sources, and runs the
+! main-loop body of main.f90:211-555 with every optional-physics flag off.
+! It is never shipped and never called by the product path; its outputs are
+! golden vectors for the C restatement (boxzy_oracle.c) and the HIP kernels.
+!
+! Usage:  ref_harness <namelist> <ic_file> <out_prefix> <max_steps> <mode> [diag]
+!   ic_file   stream-binary float64: cons(5,ngrid) u(3,ngrid) p(ngrid) adindx(ngrid)
+!             (Fortran order, exactly the reference's grid_commons arrays)
+!   mode 0    run steps; write <prefix>.dt (dt, time per step) and
+!             <prefix>.final (cons, u, p after the last step)
+!   mode 1    run steps; additionally write <prefix>.snap with the state after
+!             every sub-call of the first step (tagged records) and <prefix>.qq
+!             (avisc output on the initial velocities)
+!   diag 1    also run the per-step conservation diagnostics (main.f90:222-223)
+!
+! The step sequence below restates main.f90:211-555 with RADTRAN, COOL,
+! SELFGRAVITY, PARTICLE, EXTERNALGRAV, ENERGYCOR off and fluxangmom=.false.
+!
+program ref_harness
+ use parameters
+ use derived_types
+ use grid_commons
+ use eos
+ use input
+ use utils
+ implicit none
+
+ character(len=256)::nmlfile,icfile,prefix,arg
+ character(len=80)::nml80
+ integer::maxsteps,mode,diag,step,igrid,idim,ios
+ real(pre)::f1,dtold,tphase
+ real(pre)::etot_loc0,mass_loc0,f_loc0
+ type(units)::scale
+
+ call get_command_argument(1,nmlfile)
+ call get_command_argument(2,icfile)
+ call get_command_argument(3,prefix)
+ call get_command_argument(4,arg); read(arg,*)maxsteps
+ call get_command_argument(5,arg); read(arg,*)mode
+ diag=0
+ if(command_argument_count()>=6)then
+   call get_command_argument(6,arg); read(arg,*)diag
+ endif
+
+ nml80=nmlfile(1:80)
+ call read_params(nml80)
+ call init_grid()
+ call initialize_eos
+ call calc_eos_table
+
+! initial conditions from file, then the bookkeeping that init_conditions.f90
+! performs for every set-up: phi=0, gforce from phi (-> -0.0), cons_old=cons
+ open(unit=201,file=trim(icfile),access='stream',form='unformatted',status='old',iostat=ios)
+ if(ios/=0)stop "cannot open ic file"
+ read(201)cons
+ read(201)u
+ read(201)p
+ read(201)adindx
+ close(201)
+ phi=zero
+!$OMP PARALLEL DEFAULT(SHARED)
+ call calc_gforce()
+!$OMP END PARALLEL
+ cons_old=cons
+
+ if(mode==1)then
+   open(unit=203,file=trim(prefix)//'.snap',access='stream',form='unformatted',status='replace')
+   call snap(0)
+   ! avisc on the loaded velocities (never called by main.f90; avisc.f90:4-26)
+!$OMP PARALLEL DEFAULT(SHARED)
+   call avisc()
+!$OMP END PARALLEL
+   open(unit=204,file=trim(prefix)//'.qq',access='stream',form='unformatted',status='replace')
+   write(204)qq
+   close(204)
+ endif
+
+ open(unit=202,file=trim(prefix)//'.dt',access='stream',form='unformatted',status='replace')
+
+ call get_units(scale)
+ time=starttime
+ f1=one
+ dtold=zero
+ step=0
+
+ do
+  if(time>=endtime.or.step>=maxsteps)exit
+  step=step+1
+  etot_loc0=zero;mass_loc0=zero;f_loc0=zero
+
+!$OMP PARALLEL DEFAULT(SHARED)
+  if(diag==1)then
+    call conservation_diagnostic()
+    call get_etot(etot_loc0,mass_loc0,f_loc0)
+  endif
+  call state()
+!$OMP BARRIER
+  call courant()
+!$OMP BARRIER
+!$OMP MASTER
+  if(mode==1.and.step==1)call snap(1)
+  dt=dt*f1
+  if(step>1.and.dtold>zero)dt=min(dt,dtold*1.1d0)
+  dtold=dt
+  if(dt+time>endtime)dt=endtime-time
+  dt=dt*half
+!$OMP END MASTER
+!$OMP BARRIER
+  call source(time,0)
+!$OMP BARRIER
+!$OMP MASTER
+  if(mode==1.and.step==1)call snap(2)
+!$OMP END MASTER
+!$OMP BARRIER
+  call set_ghost_cells()
+!$OMP BARRIER
+  call state()
+!$OMP BARRIER
+!$OMP MASTER
+  if(mode==1.and.step==1)call snap(3)
+!$OMP END MASTER
+!$OMP BARRIER
+  call velocity()
+!$OMP BARRIER
+!$OMP DO SCHEDULE(STATIC)
+  do igrid=1,ngrid
+   do idim=1,5
+    cons_old(idim,igrid)=cons(idim,igrid)
+   enddo
+  enddo
+!$OMP ENDDO
+!$OMP END PARALLEL
+  if(mode==1.and.step==1)call snap(4)
+
+  dt=dt*two
+  call flux(0)
+  if(mode==1.and.step==1)call snap(5)
+
+!$OMP PARALLEL DEFAULT(SHARED)
+  call set_ghost_cells()
+!$OMP BARRIER
+  call state()
+!$OMP BARRIER
+  call velocity()
+!$OMP END PARALLEL
+  if(mode==1.and.step==1)call snap(6)
+
+  call flux(1)
+  if(mode==1.and.step==1)call snap(7)
+
+!$OMP PARALLEL DEFAULT(SHARED)
+  call set_ghost_cells()
+!$OMP BARRIER
+  call state()
+!$OMP BARRIER
+  call velocity()
+!$OMP END PARALLEL
+  if(mode==1.and.step==1)call snap(8)
+
+  dt=half*dt
+!$OMP PARALLEL DEFAULT(SHARED)
+  tphase=time+dt
+  call source(tphase,1)
+  call set_ghost_cells()
+!$OMP BARRIER
+  call state()
+!$OMP BARRIER
+  call velocity()
+!$OMP END PARALLEL
+  if(mode==1.and.step==1)call snap(9)
+
+  dt=dt*two
+  time=time+dt
+  write(202)dt,time
+ enddo
+ close(202)
+ if(mode==1)close(203)
+
+ open(unit=205,file=trim(prefix)//'.final',access='stream',form='unformatted',status='replace')
+ write(205)cons
+ write(205)u
+ write(205)p
+ close(205)
+
+contains
+
+ subroutine snap(tag)
+  integer,intent(in)::tag
+  write(203)tag
+  write(203)dt
+  write(203)cons
+  write(203)u
+  write(203)p
+  write(203)cons_old
+ end subroutine
+
+end program
