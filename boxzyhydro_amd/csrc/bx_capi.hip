@@ -1,0 +1,619 @@
+// bx_capi.hip -- the extern "C" entry points of libboxzy_hip.so (include/boxzy.h):
+// context set-up (the device counterpart of init_grid.f90), host<->device
+// transfers in the reference's Fortran layout, one entry point per reference
+// subroutine, and the step driver of main.f90:211-555.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "bx_internal.h"
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+namespace bx {
+
+int set_error(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int hip_check(hipError_t e, const char *what)
+{
+    return set_error(BX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Host-side restatement of one set_ghost_cells call (utils.f90:200-403) over
+// symbolic values: each cell holds (source cell, momentum sign flips), and the
+// ghost list is walked in the reference's serial order (the OMP DO of
+// utils.f90:314 is order-dependent only where a donor is itself a ghost, and
+// every call site repeats the call, which settles that order).  The result is
+// a gather table: ghost <- +-(value of a cell before the call).
+static int build_ghost_table(const bx_params *prm, std::vector<int32_t> &dst, std::vector<int32_t> &src,
+                             std::vector<int32_t> &sgn, std::vector<int32_t> &scratch_src)
+{
+    const int nx = prm->nx, ny = prm->ny, nz = prm->nz;
+    const int64_t n = (int64_t)nx * ny * nz;
+    auto id = [&](int ix, int iy, int iz) -> int32_t {
+        return (int32_t)((ix - 1) + (int64_t)(iy - 1) * nx + (int64_t)(iz - 1) * nx * ny);
+    };
+    auto bnd = [&](int ix, int iy, int iz) -> int {
+        bool f1 = ix == 1 || ix == nx || iy == 1 || iy == ny || (nz >= 6 && (iz == 1 || iz == nz));
+        if (f1) return 1;
+        bool f2 = ix == 2 || ix == nx - 1 || iy == 2 || iy == ny - 1 || (nz >= 6 && (iz == 2 || iz == nz - 1));
+        return f2 ? 2 : 0;
+    };
+    // utils.f90:200-281 (iz>=6 test at line 215 kept)
+    auto check_corner = [&](int ix, int iy, int iz, int32_t ineigh) -> int32_t {
+        if (ix < 3) {
+            if (iy < 3) {
+                if (nz >= 6) {
+                    if (iz < 3) ineigh = id(3, 3, 3);
+                    else if (iz > nz - 2) ineigh = id(3, 3, nz - 2);
+                    else ineigh = id(3, 3, iz);
+                } else ineigh = id(3, 3, iz);
+            } else if (iy > ny - 2) {
+                if (iz >= 6) {
+                    if (iz < 3) ineigh = id(3, ny - 2, 3);
+                    else if (iz > nz - 2) ineigh = id(3, ny - 2, nz - 2);
+                    else ineigh = id(3, ny - 2, iz);
+                } else ineigh = id(3, ny - 2, iz);
+            } else {
+                if (nz >= 6) {
+                    if (iz < 3) ineigh = id(3, iy, 3);
+                    else if (iz > nz - 2) ineigh = id(3, iy, nz - 2);
+                } else ineigh = id(3, iy, iz);
+            }
+        } else if (ix > nx - 2) {
+            if (iy < 3) {
+                if (nz >= 6) {
+                    if (iz < 3) ineigh = id(nx - 2, 3, 3);
+                    else if (iz > nz - 2) ineigh = id(nx - 2, 3, nz - 2);
+                    else ineigh = id(nx - 2, 3, iz);
+                } else ineigh = id(nx - 2, 3, iz);
+            } else if (iy > ny - 2) {
+                if (nz >= 6) {
+                    if (iz < 3) ineigh = id(nx - 2, ny - 2, 3);
+                    else if (iz > nz - 2) ineigh = id(nx - 2, ny - 2, nz - 2);
+                    else ineigh = id(nx - 2, ny - 2, iz);
+                } else ineigh = id(nx - 2, ny - 2, iz);
+            } else {
+                if (nz >= 6) {
+                    if (iz < 3) ineigh = id(nx - 2, iy, 3);
+                    else if (iz > nz - 2) ineigh = id(nx - 2, iy, nz - 2);
+                } else ineigh = id(nx - 2, iy, iz);
+            }
+        } else {
+            if (iy < 3) {
+                if (nz >= 6) {
+                    if (iz < 3) ineigh = id(ix, 3, 3);
+                    else if (iz > nz - 2) ineigh = id(ix, 3, nz - 2);
+                } else ineigh = id(ix, 3, iz);
+            } else if (iy > ny - 2) {
+                if (nz >= 6) {
+                    if (iz < 3) ineigh = id(ix, ny - 2, 3);
+                    else if (iz > nz - 2) ineigh = id(ix, ny - 2, nz - 2);
+                } else ineigh = id(ix, ny - 2, iz);
+            }
+        }
+        return ineigh;
+    };
+
+    // ghost list in cell order (init_grid.f90:166-206)
+    std::vector<int32_t> ghosts;
+    for (int iz = 1; iz <= nz; iz++)
+        for (int iy = 1; iy <= ny; iy++)
+            for (int ix = 1; ix <= nx; ix++)
+                if (bnd(ix, iy, iz) > 0) ghosts.push_back(id(ix, iy, iz));
+
+    std::vector<int32_t> vsrc(n);
+    std::vector<uint8_t> vsg(n, 0);
+    for (int64_t k = 0; k < n; k++) vsrc[k] = (int32_t)k;
+    const int32_t *nof = prm->no_outflow;
+    const bool zwall = nz >= 6;  // init_grid.f90:76-79
+    auto pass = [&]() {
+        int32_t ineigh = 0;
+        for (int32_t g : ghosts) {
+            int ix = g % nx + 1, iy = (g / nx) % ny + 1, iz = g / (nx * ny) + 1;
+            const int b = bnd(ix, iy, iz);
+            const int32_t pl = nx * ny;
+            if (b == 1) {
+                if (ix == 1) ineigh = g + 2;
+                else if (ix == nx) ineigh = g - 2;
+                else if (iy == 1) ineigh = g + 2 * nx;
+                else if (iy == ny) ineigh = g - 2 * nx;
+                else if (iz == 1) ineigh = nz < 6 ? g : g + 2 * pl;
+                else ineigh = nz < 6 ? g : g - 2 * pl;
+            } else {
+                if (ix == 2) ineigh = g + 1;
+                else if (ix == nx - 1) ineigh = g - 1;
+                else if (iy == 2) ineigh = g + nx;
+                else if (iy == ny - 1) ineigh = g - nx;
+                else if (iz == 2) ineigh = nz < 6 ? g : g + pl;
+                else if (iz == nz - 1) ineigh = nz < 6 ? g : g - pl;
+            }
+            ineigh = check_corner(ix, iy, iz, ineigh);
+            vsrc[g] = vsrc[ineigh];
+            vsg[g] = vsg[ineigh];
+            auto mirror = [&](int32_t ip, int bit) {
+                vsrc[g] = vsrc[ip];
+                vsg[g] = (uint8_t)(vsg[ip] ^ (1u << bit));
+            };
+            if (nof[0]) {
+                if (ix == 1) mirror(id(ix + 3, iy, iz), 0);
+                else if (ix == 2) mirror(id(ix + 1, iy, iz), 0);
+            }
+            if (nof[1]) {
+                if (ix == nx) mirror(id(ix - 3, iy, iz), 0);
+                else if (ix == nx - 1) mirror(id(ix - 1, iy, iz), 0);
+            }
+            if (nof[2]) {
+                if (iy == 1) mirror(id(ix, iy + 3, iz), 1);
+                else if (iy == 2) mirror(id(ix, iy + 1, iz), 1);
+            }
+            if (nof[3]) {
+                if (iy == ny) mirror(id(ix, iy - 3, iz), 1);
+                else if (iy == ny - 1) mirror(id(ix, iy - 1, iz), 1);
+            }
+            if (nof[4] && zwall) {
+                if (iz == 1) mirror(id(ix, iy, iz + 3), 2);
+                else if (iz == 2) mirror(id(ix, iy, iz + 1), 2);
+            }
+            if (nof[5] && zwall) {
+                if (iz == nz) mirror(id(ix, iy, iz - 3), 2);
+                else if (iz == nz - 1) mirror(id(ix, iy, iz - 1), 2);
+            }
+        }
+    };
+    pass();
+    // Donors that are themselves ghosts (read before the pass reached them)
+    // refer to their value from before the call; those values are snapshotted
+    // into a scratch list first.
+    dst.clear();
+    src.clear();
+    sgn.clear();
+    scratch_src.clear();
+    std::vector<int32_t> slot(n, -1);
+    for (int32_t g : ghosts) {
+        int32_t s = vsrc[g];
+        int sx = s % nx + 1, sy = (s / nx) % ny + 1, sz = s / (nx * ny) + 1;
+        if (bnd(sx, sy, sz) != 0) {
+            if (slot[s] < 0) {
+                slot[s] = (int32_t)scratch_src.size();
+                scratch_src.push_back(s);
+            }
+            s = -(slot[s] + 1);
+        }
+        dst.push_back(g);
+        src.push_back(s);
+        sgn.push_back(vsg[g]);
+    }
+    return BX_OK;
+}
+
+}  // namespace bx
+
+using namespace bx;
+
+extern "C" {
+
+const char *bx_last_error(void) { return g_last_error.c_str(); }
+
+int bx_create(const bx_params *prm, int32_t device, bx_ctx **out)
+{
+    if (!prm || !out) return set_error(BX_ERR_ARG, "bx_create: null argument");
+    *out = nullptr;
+    if (prm->nx < 5 || prm->ny < 5 || prm->nz < 1)
+        return set_error(BX_ERR_ARG, "bx_create: need nx,ny >= 5 and nz >= 1 (two ghost layers per side)");
+    if (!(prm->dx > 0 && prm->dy > 0 && prm->dz > 0)) return set_error(BX_ERR_ARG, "bx_create: spacings must be > 0");
+    if (!(prm->gammafix > 1.0)) return set_error(BX_ERR_ARG, "bx_create: gammafix must be > 1");
+    if (prm->precision != BX_PREC_FP64 && prm->precision != BX_PREC_MIXED)
+        return set_error(BX_ERR_ARG, "bx_create: unknown precision");
+    const int64_t n = (int64_t)prm->nx * prm->ny * prm->nz;
+    if (n > (int64_t)INT32_MAX) return set_error(BX_ERR_ARG, "bx_create: grid exceeds 2^31 cells");
+
+    int ndev = 0;
+    BX_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return set_error(BX_ERR_ARG, "bx_create: no such device");
+    BX_HIP(hipSetDevice(device));
+
+    bx_ctx *c = new (std::nothrow) bx_ctx();
+    if (!c) return set_error(BX_ERR_ARG, "bx_create: out of host memory");
+    c->prm = *prm;
+    c->device = device;
+    c->nx = prm->nx;
+    c->ny = prm->ny;
+    c->nz = prm->nz;
+    c->n = n;
+    c->flatz = prm->nz < 6;
+    if (c->flatz) {
+        c->prm.no_outflow[4] = 0;
+        c->prm.no_outflow[5] = 0;
+    }
+    c->muc = 1.0 / (prm->xabun * 0.5 + prm->yabun * 0.25 + prm->zabun / prm->mu_z);  // eos.f90:144
+    {
+        const double pi = 3.1415926535897931;  // units.f90:10-17
+        const double time = (3600.0 * 365.25 * 24.0) / (2.0 * pi);
+        const double vel = 1.496e13 / time;
+        c->rgas = 8.254e7 / ((vel * vel) / 1.0);
+    }
+    // calc_gforce with phi=0 (utils.f90:566-568): -(0-0)/(2*d) = -0.0
+    c->grav[0] = -((0.0 - 0.0) / (2.0 * prm->dx));
+    c->grav[1] = -((0.0 - 0.0) / (2.0 * prm->dy));
+    c->grav[2] = -((0.0 - 0.0) / (2.0 * prm->dz));
+
+    std::vector<int32_t> gd, gs, gg, gx;
+    int rc = build_ghost_table(&c->prm, gd, gs, gg, gx);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    c->nghost = (int64_t)gd.size();
+    c->nscratch = (int64_t)gx.size();
+
+    auto fail = [&](hipError_t e, const char *what) {
+        int r = hip_check(e, what);
+        bx_destroy(c);
+        return r;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail(e, "hipStreamCreate");
+    const size_t pb = sizeof(double) * (size_t)n;
+    struct {
+        double **ptr;
+        size_t planes;
+    } allocs[] = {{&c->cons, 5}, {&c->cons_old, 5}, {&c->u, 3}, {&c->p, 1},
+                  {&c->adindx, 1}, {&c->qq, 3}};
+    for (auto &a : allocs) {
+        if ((e = hipMalloc((void **)a.ptr, pb * a.planes)) != hipSuccess) return fail(e, "hipMalloc state");
+        if ((e = hipMemsetAsync(*a.ptr, 0, pb * a.planes, c->stream)) != hipSuccess) return fail(e, "hipMemset");
+    }
+    const size_t ng = std::max<size_t>(1, gd.size());
+    if ((e = hipMalloc((void **)&c->ghost_dst, sizeof(int32_t) * ng)) != hipSuccess) return fail(e, "hipMalloc ghosts");
+    if ((e = hipMalloc((void **)&c->ghost_src, sizeof(int32_t) * ng)) != hipSuccess) return fail(e, "hipMalloc ghosts");
+    if ((e = hipMalloc((void **)&c->ghost_sign, sizeof(int32_t) * ng)) != hipSuccess) return fail(e, "hipMalloc ghosts");
+    if (!gd.empty()) {
+        if ((e = hipMemcpy(c->ghost_dst, gd.data(), sizeof(int32_t) * gd.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "hipMemcpy ghosts");
+        if ((e = hipMemcpy(c->ghost_src, gs.data(), sizeof(int32_t) * gs.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "hipMemcpy ghosts");
+        if ((e = hipMemcpy(c->ghost_sign, gg.data(), sizeof(int32_t) * gg.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "hipMemcpy ghosts");
+    }
+    if (!gx.empty()) {
+        if ((e = hipMalloc((void **)&c->ghost_scratch_src, sizeof(int32_t) * gx.size())) != hipSuccess)
+            return fail(e, "hipMalloc ghost scratch");
+        if ((e = hipMalloc((void **)&c->ghost_scratch, sizeof(double) * 5 * gx.size())) != hipSuccess)
+            return fail(e, "hipMalloc ghost scratch");
+        if ((e = hipMemcpy(c->ghost_scratch_src, gx.data(), sizeof(int32_t) * gx.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+            return fail(e, "hipMemcpy ghost scratch");
+    }
+    if ((e = hipMalloc((void **)&c->dsc, sizeof(DevScalars))) != hipSuccess) return fail(e, "hipMalloc scalars");
+    if ((e = hipMemsetAsync(c->dsc, 0, sizeof(DevScalars), c->stream)) != hipSuccess) return fail(e, "hipMemset");
+    if ((e = hipHostMalloc((void **)&c->hsc, sizeof(DevScalars), hipHostMallocDefault)) != hipSuccess)
+        return fail(e, "hipHostMalloc");
+    std::memset(c->hsc, 0, sizeof(DevScalars));
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail(e, "hipStreamSynchronize");
+    *out = c;
+    return BX_OK;
+}
+
+int bx_destroy(bx_ctx *c)
+{
+    if (!c) return BX_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *dev[] = {c->cons, c->cons_old, c->u, c->p, c->adindx, c->qq, c->states, c->ghost_dst, c->ghost_src,
+                   c->ghost_sign, c->ghost_scratch_src, c->ghost_scratch, c->ustar, c->pstar, c->unext, c->dsc,
+                   c->cons32, c->ustar32, c->unext32};
+    for (void *p : dev)
+        if (p) (void)hipFree(p);
+    if (c->hsc) (void)hipHostFree(c->hsc);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BX_OK;
+}
+
+int bx_ghost_table(const bx_params *prm, int64_t *nghost, int64_t *nscratch, int32_t *dst, int32_t *src,
+                   int32_t *sign, int32_t *scratch_src)
+{
+    if (!prm || !nghost || !nscratch) return set_error(BX_ERR_ARG, "bx_ghost_table: null argument");
+    if (prm->nx < 5 || prm->ny < 5 || prm->nz < 1) return set_error(BX_ERR_ARG, "bx_ghost_table: bad grid");
+    bx_params p = *prm;
+    if (p.nz < 6) p.no_outflow[4] = p.no_outflow[5] = 0;
+    std::vector<int32_t> gd, gs, gg, gx;
+    int rc = build_ghost_table(&p, gd, gs, gg, gx);
+    if (rc) return rc;
+    *nghost = (int64_t)gd.size();
+    *nscratch = (int64_t)gx.size();
+    if (dst) std::copy(gd.begin(), gd.end(), dst);
+    if (src) std::copy(gs.begin(), gs.end(), src);
+    if (sign) std::copy(gg.begin(), gg.end(), sign);
+    if (scratch_src) std::copy(gx.begin(), gx.end(), scratch_src);
+    return BX_OK;
+}
+
+int bx_ngrid(const bx_ctx *c, int64_t *ngrid, int64_t *nghost)
+{
+    if (!c) return set_error(BX_ERR_ARG, "bx_ngrid: null context");
+    if (ngrid) *ngrid = c->n;
+    if (nghost) *nghost = c->nghost;
+    return BX_OK;
+}
+
+int bx_sync(bx_ctx *c)
+{
+    if (!c) return set_error(BX_ERR_ARG, "bx_sync: null context");
+    BX_HIP(hipSetDevice(c->device));
+    BX_HIP(hipStreamSynchronize(c->stream));
+    return BX_OK;
+}
+
+// AoS (k, n) host array <-> k SoA device planes
+static int upload_aos(bx_ctx *c, const double *h, int k, double *d)
+{
+    const int64_t n = c->n;
+    std::vector<double> tmp((size_t)n * k);
+    for (int64_t i = 0; i < n; i++)
+        for (int f = 0; f < k; f++) tmp[(size_t)f * n + i] = h[(size_t)i * k + f];
+    BX_HIP(hipMemcpyAsync(d, tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice, c->stream));
+    BX_HIP(hipStreamSynchronize(c->stream));
+    return BX_OK;
+}
+
+static int download_aos(bx_ctx *c, const double *d, int k, double *h)
+{
+    const int64_t n = c->n;
+    std::vector<double> tmp((size_t)n * k);
+    BX_HIP(hipMemcpyAsync(tmp.data(), d, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, c->stream));
+    BX_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < n; i++)
+        for (int f = 0; f < k; f++) h[(size_t)i * k + f] = tmp[(size_t)f * n + i];
+    return BX_OK;
+}
+
+int bx_upload(bx_ctx *c, const double *cons, const double *u, const double *p, const double *adindx)
+{
+    if (!c) return set_error(BX_ERR_ARG, "bx_upload: null context");
+    BX_HIP(hipSetDevice(c->device));
+    int rc;
+    if (cons) {
+        if ((rc = upload_aos(c, cons, 5, c->cons))) return rc;
+        BX_HIP(hipMemcpyAsync(c->cons_old, c->cons, sizeof(double) * 5 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (u && (rc = upload_aos(c, u, 3, c->u))) return rc;
+    if (p && (rc = upload_aos(c, p, 1, c->p))) return rc;
+    if (adindx && (rc = upload_aos(c, adindx, 1, c->adindx))) return rc;
+    BX_HIP(hipStreamSynchronize(c->stream));
+    c->fused_valid = false;
+    c->aux_stale = false;
+    c->old_stale = false;
+    return BX_OK;
+}
+
+int bx_download(bx_ctx *c, double *cons, double *u, double *p, double *adindx)
+{
+    if (!c) return set_error(BX_ERR_ARG, "bx_download: null context");
+    BX_HIP(hipSetDevice(c->device));
+    int rc;
+    if (c->aux_stale && (u || p || adindx)) {
+        if ((rc = launch_materialize(c))) return rc;
+        c->aux_stale = false;
+    }
+    if (cons && (rc = download_aos(c, c->cons, 5, cons))) return rc;
+    if (u && (rc = download_aos(c, c->u, 3, u))) return rc;
+    if (p && (rc = download_aos(c, c->p, 1, p))) return rc;
+    if (adindx && (rc = download_aos(c, c->adindx, 1, adindx))) return rc;
+    return BX_OK;
+}
+
+int bx_download_cons_old(bx_ctx *c, double *cons_old)
+{
+    if (!c || !cons_old) return set_error(BX_ERR_ARG, "bx_download_cons_old: null argument");
+    if (c->old_stale)
+        return set_error(BX_ERR_STATE, "bx_download_cons_old: cons_old is not kept by the fused step");
+    BX_HIP(hipSetDevice(c->device));
+    return download_aos(c, c->cons_old, 5, cons_old);
+}
+
+// The per-subroutine entry points operate on the full reference state; after
+// fused steps the velocity and pressure planes are first rebuilt from cons.
+static int ensure_aux(bx_ctx *c)
+{
+    BX_HIP(hipSetDevice(c->device));
+    if (c->aux_stale) {
+        int rc = launch_materialize(c);
+        if (rc) return rc;
+        c->aux_stale = false;
+    }
+    c->fused_valid = false;
+    return BX_OK;
+}
+
+int bx_state(bx_ctx *c)
+{
+    if (!c) return set_error(BX_ERR_ARG, "bx_state: null context");
+    int rc = ensure_aux(c);
+    return rc ? rc : launch_state(c);
+}
+
+int bx_velocity(bx_ctx *c, double *maxv)
+{
+    if (!c) return set_error(BX_ERR_ARG, "bx_velocity: null context");
+    int rc = ensure_aux(c);
+    if (rc || (rc = launch_velocity(c, maxv != nullptr))) return rc;
+    if (maxv) {
+        BX_HIP(hipMemcpyAsync(&c->hsc->maxv_bits, &c->dsc->maxv_bits, sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, c->stream));
+        BX_HIP(hipStreamSynchronize(c->stream));
+        unsigned long long b = c->hsc->maxv_bits;
+        std::memcpy(maxv, &b, sizeof(double));
+    }
+    return BX_OK;
+}
+
+int bx_courant(bx_ctx *c, double *dt)
+{
+    if (!c || !dt) return set_error(BX_ERR_ARG, "bx_courant: null argument");
+    int rc = ensure_aux(c);
+    if (rc || (rc = launch_courant(c))) return rc;
+    BX_HIP(hipMemcpyAsync(&c->hsc->rate_bits[0], &c->dsc->rate_bits[0], sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, c->stream));
+    BX_HIP(hipStreamSynchronize(c->stream));
+    double rate;
+    unsigned long long b = c->hsc->rate_bits[0];
+    std::memcpy(&rate, &b, sizeof(double));
+    *dt = c->prm.cfl / rate;  // courant.f90:29
+    return BX_OK;
+}
+
+int bx_source(bx_ctx *c, double tphase, int32_t siter, double dt)
+{
+    (void)tphase;  // used only by EXTERNALGRAV/ROTATE builds of the reference
+    (void)siter;
+    if (!c) return set_error(BX_ERR_ARG, "bx_source: null context");
+    int rc = ensure_aux(c);
+    return rc ? rc : launch_source(c, dt);
+}
+
+int bx_set_ghost_cells(bx_ctx *c)
+{
+    if (!c) return set_error(BX_ERR_ARG, "bx_set_ghost_cells: null context");
+    int rc = ensure_aux(c);
+    return rc ? rc : launch_ghosts(c, c->cons);
+}
+
+int bx_flux(bx_ctx *c, int32_t avg, double dt)
+{
+    if (!c) return set_error(BX_ERR_ARG, "bx_flux: null context");
+    if (avg != 0 && avg != 1) return set_error(BX_ERR_ARG, "bx_flux: avg must be 0 or 1");
+    int rc = ensure_aux(c);
+    if (rc) return rc;
+    if (!c->states) BX_HIP(hipMalloc((void **)&c->states, sizeof(double) * 42 * (size_t)c->n));
+    return launch_flux(c, avg, dt);
+}
+
+int bx_copy_old(bx_ctx *c)
+{
+    if (!c) return set_error(BX_ERR_ARG, "bx_copy_old: null context");
+    int rc = ensure_aux(c);
+    if (rc) return rc;
+    BX_HIP(hipMemcpyAsync(c->cons_old, c->cons, sizeof(double) * 5 * c->n, hipMemcpyDeviceToDevice, c->stream));
+    c->old_stale = false;
+    return BX_OK;
+}
+
+int bx_avisc(bx_ctx *c, double *qq)
+{
+    if (!c) return set_error(BX_ERR_ARG, "bx_avisc: null context");
+    int rc = ensure_aux(c);
+    if (rc || (rc = launch_avisc(c))) return rc;
+    if (qq) return download_aos(c, c->qq, 3, qq);
+    return BX_OK;
+}
+
+int bx_fallback_count(const bx_ctx *c, int64_t *count)
+{
+    if (!c || !count) return set_error(BX_ERR_ARG, "bx_fallback_count: null argument");
+    *count = c->fallbacks;
+    return BX_OK;
+}
+
+}  // extern "C"
+
+namespace bx {
+
+// The second of each pair of set_ghost_cells calls; a no-op when every donor
+// is an interior cell (one call then already gives the fixed point).
+static int ghosts_again(bx_ctx *c) { return c->nscratch > 0 ? launch_ghosts(c, c->cons) : BX_OK; }
+
+// One step of main.f90:211-555 with the reference's kernel sequence.
+// dt logic (main.f90:266-281) runs on the host in the same IEEE operations.
+static int exact_step(bx_ctx *c, double endtime, double *time, double *dtold, int32_t *step, double *dt_out)
+{
+    int rc;
+    if (!c->states) BX_HIP(hipMalloc((void **)&c->states, sizeof(double) * 42 * (size_t)c->n));
+    *step += 1;
+    if ((rc = launch_state(c))) return rc;  // :232
+    double dt;
+    {
+        if ((rc = launch_courant(c))) return rc;  // :241
+        BX_HIP(hipMemcpyAsync(&c->hsc->rate_bits[0], &c->dsc->rate_bits[0], sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, c->stream));
+        BX_HIP(hipStreamSynchronize(c->stream));
+        double rate;
+        unsigned long long b = c->hsc->rate_bits[0];
+        std::memcpy(&rate, &b, sizeof(double));
+        dt = c->prm.cfl / rate;
+    }
+    const double f1 = 1.0;
+    dt = dt * f1;
+    if (*step > 1 && *dtold > 0.0) dt = std::min(dt, *dtold * 1.1);
+    *dtold = dt;
+    if (dt + *time > endtime) dt = endtime - *time;
+    dt = dt * 0.5;
+    if ((rc = launch_source(c, dt))) return rc;  // :289 (+ ghosts, source.f90:199)
+    if ((rc = ghosts_again(c))) return rc;       // :312
+    if ((rc = launch_state(c))) return rc;       // :314
+    if ((rc = launch_velocity(c, false))) return rc;  // :318
+    BX_HIP(hipMemcpyAsync(c->cons_old, c->cons, sizeof(double) * 5 * c->n, hipMemcpyDeviceToDevice,
+                          c->stream));  // :327-333
+    dt = dt * 2.0;
+    if ((rc = launch_flux(c, 0, dt))) return rc;  // :346 (+ ghosts, flux.f90:231)
+    if ((rc = ghosts_again(c))) return rc;        // :351
+    if ((rc = launch_state(c))) return rc;
+    if ((rc = launch_velocity(c, false))) return rc;
+    if ((rc = launch_flux(c, 1, dt))) return rc;  // :368 (+ ghosts, flux.f90:231)
+    if ((rc = ghosts_again(c))) return rc;        // :374
+    if ((rc = launch_state(c))) return rc;
+    if ((rc = launch_velocity(c, false))) return rc;
+    dt = 0.5 * dt;
+    if ((rc = launch_source(c, dt))) return rc;  // :490 (+ ghosts, source.f90:199)
+    if ((rc = ghosts_again(c))) return rc;       // :511
+    if ((rc = launch_state(c))) return rc;
+    if ((rc = launch_velocity(c, false))) return rc;
+    dt = dt * 2.0;
+    *time = *time + dt;
+    *dt_out = dt;
+    c->old_stale = false;
+    return BX_OK;
+}
+
+}  // namespace bx
+
+extern "C" int bx_step(bx_ctx *c, int32_t mode, int32_t nsteps, double endtime, double *time, double *dtold,
+                       int32_t *step, double *dts, int32_t *taken)
+{
+    if (!c || !time || !dtold || !step) return set_error(BX_ERR_ARG, "bx_step: null argument");
+    if (mode != BX_STEP_EXACT && mode != BX_STEP_FUSED) return set_error(BX_ERR_ARG, "bx_step: unknown mode");
+    BX_HIP(hipSetDevice(c->device));
+    int32_t k = 0;
+    int rc = BX_OK;
+    if (mode == BX_STEP_EXACT || c->prm.precision != BX_PREC_FP64) {
+        if ((rc = ensure_aux(c))) return rc;
+        if (c->prm.precision != BX_PREC_FP64)
+            return set_error(BX_ERR_UNSUPPORTED, "bx_step: the exact sequence is FP64 only");
+        while (k < nsteps && *time < endtime && *step < 1000000) {
+            double dt;
+            if ((rc = exact_step(c, endtime, time, dtold, step, &dt))) break;
+            if (dts) dts[k] = dt;
+            k++;
+        }
+    } else {
+        return set_error(BX_ERR_UNSUPPORTED, "bx_step: fused mode not built yet");
+    }
+    if (taken) *taken = k;
+    if (rc) return rc;
+    BX_HIP(hipStreamSynchronize(c->stream));
+    return BX_OK;
+}
+
+namespace bx {
+int launch_materialize(bx_ctx *) { return set_error(BX_ERR_UNSUPPORTED, "materialize: not built yet"); }
+}  // namespace bx

@@ -1,0 +1,447 @@
+// bx_exact.hip -- the hydro step as the reference structures it: one kernel per
+// reference subroutine, acting on device-resident SoA planes.  These kernels
+// are the plain FP64 drop-in behind bx_state/bx_velocity/bx_courant/
+// bx_source/bx_set_ghost_cells/bx_flux/bx_avisc, and the exact fallback of
+// the fused step.  Arithmetic order follows the cited reference lines so the
+// results are bit-identical to the reference's FP64 step.
+#include "bx_internal.h"
+#include "bx_math.h"
+
+namespace bx {
+
+constexpr int kThreads = 256;
+
+// 1-based (ix,iy,iz) of cell c (utils.f90:191-194 inverse)
+__device__ __forceinline__ void coords(int64_t c, int nx, int ny, int &ix, int &iy, int &iz)
+{
+    int64_t plane = (int64_t)nx * ny;
+    iz = (int)(c / plane) + 1;
+    int64_t r = c - (int64_t)(iz - 1) * plane;
+    iy = (int)(r / nx) + 1;
+    ix = (int)(r - (int64_t)(iy - 1) * nx) + 1;
+}
+
+// init_grid.f90:81-107 ghost flag: 1 outer layer, 2 second layer, 0 interior
+__device__ __forceinline__ int boundary_of(int ix, int iy, int iz, int nx, int ny, int nz)
+{
+    int f1 = (ix == 1 || ix == nx) || (iy == 1 || iy == ny) || (nz >= 6 && (iz == 1 || iz == nz));
+    if (f1) return 1;
+    int f2 = (ix == 2 || ix == nx - 1) || (iy == 2 || iy == ny - 1) || (nz >= 6 && (iz == 2 || iz == nz - 1));
+    return f2 ? 2 : 0;
+}
+
+// max-reduce non-negative doubles of a block into *dst (as IEEE bits)
+__device__ __forceinline__ void block_max_to(double v, unsigned long long *dst)
+{
+    __shared__ double wmax[kThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) v = vmax(v, __shfl_xor(v, off, 64));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wmax[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmax[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); k++) m = vmax(m, wmax[k]);
+        atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// ------------------------------------------------------------------ state
+// state.f90:109-139, H2STAT=-1 branch
+__global__ __launch_bounds__(kThreads) void k_state(int64_t n, double *__restrict__ cons,
+                                                   const double *__restrict__ u, double *__restrict__ p,
+                                                   double *__restrict__ ad, double gam, double seps,
+                                                   double muc, double rgas)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double rho = cons[c], E = cons[4 * n + c];
+    const double u0 = u[c], u1 = u[n + c], u2 = u[2 * n + c];
+    const double ekin = rho * 0.5 * (u0 * u0 + u1 * u1 + u2 * u2);
+    double eps = vmax(E - ekin, seps);
+    ad[c] = gam;
+    if (rho < 1e-4) {
+        const double tk = vmin(5e3, p[c] / (rho * rgas) * muc);
+        const double pc = rho * tk * rgas / muc;
+        p[c] = pc;
+        eps = pc / (gam - 1.0);
+        cons[4 * n + c] = eps + ekin;
+    } else {
+        p[c] = eps * (gam - 1.0);
+    }
+}
+
+// --------------------------------------------------------------- velocity
+// velocity.f90:18-53: u = m/rho, maxv over |u| before the clamp, vlimit clamp
+__global__ __launch_bounds__(kThreads) void k_velocity(int64_t n, double *__restrict__ cons,
+                                                      double *__restrict__ u, double vlimit,
+                                                      double seps, unsigned long long *maxv)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double vm = 0.0;
+    if (c < n) {
+        const double rho = cons[c];
+        double m[3], v[3];
+        for (int d = 0; d < 3; d++) {
+            m[d] = cons[(1 + d) * n + c];
+            v[d] = m[d] / rho;
+        }
+        int flag = 0;
+        for (int d = 0; d < 3; d++) {
+            vm = vmax(vm, fabs(v[d]));
+            if (fabs(v[d]) > vlimit) flag = 1;
+        }
+        if (flag) {
+            double E = cons[4 * n + c];
+            double ekin = 0.5 * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) / rho;
+            double speed = 0.0;
+            for (int d = 0; d < 3; d++) speed = speed + v[d] * v[d];
+            speed = sqrt(speed);
+            for (int d = 0; d < 3; d++) {
+                v[d] = v[d] * vlimit / speed;
+                m[d] = v[d] * rho;
+                cons[(1 + d) * n + c] = m[d];
+            }
+            E = E - ekin;
+            ekin = 0.5 * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) / rho;
+            cons[4 * n + c] = vmax(E, seps) + ekin;
+        }
+        for (int d = 0; d < 3; d++) u[d * n + c] = v[d];
+    }
+    if (maxv) block_max_to(vm, maxv);
+}
+
+// ---------------------------------------------------------------- courant
+// courant.f90:17-29: max over all cells of sum_d (c+|u_d|)/Delta_d
+__global__ __launch_bounds__(kThreads) void k_courant(int64_t n, const double *__restrict__ cons,
+                                                     const double *__restrict__ u,
+                                                     const double *__restrict__ p,
+                                                     const double *__restrict__ ad, double dx,
+                                                     double dy, double dz, unsigned long long *rate)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double r0 = 0.0;
+    if (c < n) {
+        const double sound = sqrt(ad[c] * p[c] / cons[c]);
+        r0 = (sound + fabs(u[c])) / dx + (sound + fabs(u[n + c])) / dy + (sound + fabs(u[2 * n + c])) / dz;
+    }
+    block_max_to(r0, rate);
+}
+
+// ----------------------------------------------------------------- source
+// source.f90:148-182 on interior cells with a uniform gravity vector g
+// (the reference's g = -(phi(+)-phi(-))/(2d) = -0.0 for phi=0)
+__global__ __launch_bounds__(kThreads) void k_source(int64_t n, int nx, int ny, int nz,
+                                                    double *__restrict__ cons,
+                                                    const double *__restrict__ u, double dt,
+                                                    double g0, double g1, double g2, double seps)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int ix, iy, iz;
+    coords(c, nx, ny, ix, iy, iz);
+    if (boundary_of(ix, iy, iz, nx, ny, nz) > 0) return;
+    const double rho = cons[c];
+    double m0 = cons[n + c], m1 = cons[2 * n + c], m2 = cons[3 * n + c];
+    const double ux = u[c], uy = u[n + c], uz = u[2 * n + c];
+    m1 = m1 + (rho * g1) * dt;
+    m0 = m0 + (rho * g0) * dt;
+    m2 = m2 + (g2 * rho) * dt;
+    const double ekin = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / rho;
+    const double E = cons[4 * n + c];
+    cons[n + c] = m0;
+    cons[2 * n + c] = m1;
+    cons[3 * n + c] = m2;
+    cons[4 * n + c] = vmax(E - ekin + dt * rho * (ux * g0 + uy * g1 + uz * g2), seps) + ekin;
+}
+
+// ------------------------------------------------------------ ghost cells
+// utils.f90:314-403 as resolved on the host into (ghost, donor, momentum sign
+// flips).  Donors that are ghosts are read from a snapshot taken first.
+__global__ __launch_bounds__(kThreads) void k_ghost_snapshot(int64_t n, int64_t ns,
+                                                            const int32_t *__restrict__ ssrc,
+                                                            const double *__restrict__ planes,
+                                                            double *__restrict__ scratch)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ns) return;
+    const int64_t s = ssrc[k];
+    for (int f = 0; f < 5; f++) scratch[f * ns + k] = planes[f * n + s];
+}
+
+__global__ __launch_bounds__(kThreads) void k_ghosts(int64_t n, int64_t ng, const int32_t *__restrict__ dst,
+                                                    const int32_t *__restrict__ src,
+                                                    const int32_t *__restrict__ sign,
+                                                    double *__restrict__ planes,
+                                                    const double *__restrict__ scratch, int64_t ns)
+{
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ng) return;
+    const int64_t d = dst[k];
+    const int32_t s0 = src[k];
+    const double *base;
+    int64_t stride, s;
+    if (s0 >= 0) {
+        base = planes;
+        stride = n;
+        s = s0;
+    } else {
+        base = scratch;
+        stride = ns;
+        s = -(int64_t)s0 - 1;
+    }
+    const int sg = sign[k];
+    planes[d] = base[s];
+    for (int f = 1; f <= 3; f++) {
+        double v = base[f * stride + s];
+        planes[f * n + d] = ((sg >> (f - 1)) & 1) ? -v : v;
+    }
+    planes[4 * n + d] = base[4 * stride + s];
+}
+
+// ------------------------------------------------------------ flux: states
+// flux.f90:47-116: slopes (calculate_slopes, 253-295) and interface states
+// (calculate_states, 312-482) of every cell; ghosts get zero slopes.
+// Output: 42 planes, index (dir*14 + k), k = 0..6 minus (rho,u,v,w,p,eps,gamma), 7..13 plus.
+__global__ __launch_bounds__(kThreads) void k_flux_states(int64_t n, int nx, int ny, int nz, bool flatz,
+                                                         const double *__restrict__ cons,
+                                                         const double *__restrict__ u,
+                                                         const double *__restrict__ p,
+                                                         const double *__restrict__ ad,
+                                                         double *__restrict__ st, double dt, double dx,
+                                                         double dy, double dz, double seps)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int ix, iy, iz;
+    coords(c, nx, ny, ix, iy, iz);
+    const bool ghost = boundary_of(ix, iy, iz, nx, ny, nz) > 0;
+    const int64_t stride[3] = {1, (int64_t)nx, flatz ? 0 : (int64_t)nx * ny};
+    const double ds[3] = {dx, dy, dz};
+
+    // own primitives (u from the last velocity() call, p from the last state())
+    const double rho = cons[c], E = cons[4 * n + c];
+    const double v[3] = {u[c], u[n + c], u[2 * n + c]};
+    const double pc = p[c], gc = ad[c];
+    const double eps = vmax(E - 0.5 * rho * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), seps);
+    const double sound = sqrt(pc * gc / rho);
+
+    for (int d = 0; d < 3; d++) {
+        double su[3] = {0.0, 0.0, 0.0}, sp = 0.0, sd = 0.0, se = 0.0, sgm = 0.0;
+        if (!ghost) {
+            const int64_t cm = c - stride[d], cp = c + stride[d];
+            for (int j = 0; j < 3; j++) su[j] = slope(u[j * n + cm], v[j], u[j * n + cp]);
+            double em, ep;
+            {
+                const double r = cons[cm];
+                const double a0 = u[cm], a1 = u[n + cm], a2 = u[2 * n + cm];
+                em = vmax(cons[4 * n + cm] - 0.5 * r * (a0 * a0 + a1 * a1 + a2 * a2), seps);
+            }
+            {
+                const double r = cons[cp];
+                const double a0 = u[cp], a1 = u[n + cp], a2 = u[2 * n + cp];
+                ep = vmax(cons[4 * n + cp] - 0.5 * r * (a0 * a0 + a1 * a1 + a2 * a2), seps);
+            }
+            sp = slope(p[cm], pc, p[cp]);
+            sd = slope(cons[cm], rho, cons[cp]);
+            se = slope(em, eps, ep);
+            sgm = slope(ad[cm], gc, ad[cp]);
+        }
+        const double lp = vmax(v[d] + sound, 0.0) * dt / ds[d];
+        const double lm = vmin(v[d] - sound, 0.0) * dt / ds[d];
+        const double zc = 0.0 * (dt * 0.5 / ds[d]);  // transverse terms: flux.f90:391-479
+        double mi[7], pl[7];
+        muscl(rho, lm, lp, sd, mi[0], pl[0]);
+        for (int j = 0; j < 3; j++) muscl(v[j], lm, lp, su[j], mi[1 + j], pl[1 + j]);
+        muscl(pc, lm, lp, sp, mi[4], pl[4]);
+        muscl(eps, lm, lp, se, mi[5], pl[5]);
+        muscl(gc, lm, lp, sgm, mi[6], pl[6]);
+        for (int k = 0; k < 7; k++) {
+            st[(int64_t)(d * 14 + k) * n + c] = mi[k] + zc;
+            st[(int64_t)(d * 14 + 7 + k) * n + c] = pl[k] + zc;
+        }
+    }
+}
+
+__device__ __forceinline__ Side<double> load_side(const double *__restrict__ st, int64_t n, int d, int pm,
+                                                 int64_t c)
+{
+    const double *b = st + (int64_t)(d * 14 + 7 * pm) * n + c;
+    Side<double> s;
+    s.d = b[0];
+    s.v[0] = b[n];
+    s.v[1] = b[2 * n];
+    s.v[2] = b[3 * n];
+    s.p = b[4 * n];
+    s.e = b[5 * n];
+    s.g = b[6 * n];
+    return s;
+}
+
+template <int dir>
+__device__ __forceinline__ void dir_fluxes(const double *__restrict__ st, int64_t n, int64_t c, int64_t s,
+                                           double f0[5], double f1[5])
+{
+    face_flux<double, dir>(load_side(st, n, dir, 1, c - s), load_side(st, n, dir, 0, c), f0);
+    face_flux<double, dir>(load_side(st, n, dir, 1, c), load_side(st, n, dir, 0, c + s), f1);
+}
+
+// ------------------------------------------------- flux: faces and update
+// flux.f90:121-230: six HLL faces and fluxtmp on interior cells (fluxtmp=0 on
+// ghosts, flux.f90:51), then the stage update and floors on every cell.
+__global__ __launch_bounds__(kThreads) void k_flux_faces(int64_t n, int nx, int ny, int nz, bool flatz,
+                                                        const double *__restrict__ st,
+                                                        double *__restrict__ cons,
+                                                        const double *__restrict__ cons_old, int avg,
+                                                        double dt, double ayz, double axz, double axy,
+                                                        double vol, double srho, double seps)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int ix, iy, iz;
+    coords(c, nx, ny, ix, iy, iz);
+    double ftmp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (boundary_of(ix, iy, iz, nx, ny, nz) == 0) {
+        double fx0[5], fx1[5], fy0[5], fy1[5], fz0[5], fz1[5];
+        dir_fluxes<0>(st, n, c, 1, fx0, fx1);
+        dir_fluxes<1>(st, n, c, nx, fy0, fy1);
+        dir_fluxes<2>(st, n, c, flatz ? 0 : (int64_t)nx * ny, fz0, fz1);
+        for (int i = 0; i < 5; i++)
+            ftmp[i] = -((ayz * (fx1[i] - fx0[i]) + axz * (fy1[i] - fy0[i]) + axy * (fz1[i] - fz0[i])) * dt / vol);
+    }
+    double U[5];
+    for (int i = 0; i < 5; i++) {
+        const double ft = ftmp[i];
+        const double cu = cons[(int64_t)i * n + c];
+        U[i] = avg == 0 ? cu + ft : 0.5 * (cons_old[(int64_t)i * n + c] + cu + ft);
+    }
+    U[0] = vmax(U[0], srho);
+    if (U[4] < 0.0) U[4] = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0] + seps;
+    for (int i = 0; i < 5; i++) cons[(int64_t)i * n + c] = U[i];
+}
+
+// ------------------------------------------------------------------ avisc
+// avisc.f90:13-24 with get_boundary (utils.f90:62-74).  Outer-layer cells
+// (boundary=1) have only their donor in ineigh(1) in the reference and read
+// ineigh(2:6)=0 (outside the array); here those neighbours are the cell itself.
+__global__ __launch_bounds__(kThreads) void k_avisc(int64_t n, int nx, int ny, int nz, bool flatz,
+                                                   const double *__restrict__ u, double *__restrict__ qq,
+                                                   double ax, double ay, double az)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int ix, iy, iz;
+    coords(c, nx, ny, ix, iy, iz);
+    const int bnd = boundary_of(ix, iy, iz, nx, ny, nz);
+    if (bnd == 0) {
+        qq[c] = 0.0;
+        qq[n + c] = 0.0;
+        qq[2 * n + c] = 0.0;
+        return;
+    }
+    const int64_t pl = (int64_t)nx * ny;
+    int64_t nb[6];
+    if (bnd == 2) {
+        nb[0] = c + nx;
+        nb[1] = c - nx;
+        nb[2] = c + 1;
+        nb[3] = c - 1;
+        nb[4] = flatz ? c : c + pl;
+        nb[5] = flatz ? c : c - pl;
+    } else {
+        for (int k = 0; k < 6; k++) nb[k] = c;
+        if (ix == 1) nb[0] = c + 2;
+        else if (ix == nx) nb[0] = c - 2;
+        else if (iy == 1) nb[0] = c + 2 * (int64_t)nx;
+        else if (iy == ny) nb[0] = c - 2 * (int64_t)nx;
+        else if (iz == 1) nb[0] = flatz ? c : c + 2 * pl;
+        else nb[0] = flatz ? c : c - 2 * pl;
+    }
+    for (int k = 0; k < 6; k++) {
+        int jx, jy, jz;
+        coords(nb[k], nx, ny, jx, jy, jz);
+        if (boundary_of(jx, jy, jz, nx, ny, nz) > 0) nb[k] = c;
+    }
+    double t;
+    t = vmin(0.0, u[nb[2]] - u[nb[3]]);
+    qq[c] = ax * (t * t);
+    t = vmin(0.0, u[n + nb[0]] - u[n + nb[1]]);
+    qq[n + c] = ay * (t * t);
+    t = vmin(0.0, u[2 * n + nb[4]] - u[2 * n + nb[5]]);
+    qq[2 * n + c] = az * (t * t);
+}
+
+// ------------------------------------------------------------- launchers
+int launch_state(bx_ctx *c)
+{
+    hipLaunchKernelGGL(k_state, dim3(blocks_for(c->n, kThreads)), dim3(kThreads), 0, c->stream, c->n, c->cons,
+                       c->u, c->p, c->adindx, c->prm.gammafix, c->prm.small_eps, c->muc, c->rgas);
+    BX_HIP(hipGetLastError());
+    return BX_OK;
+}
+
+int launch_velocity(bx_ctx *c, bool want_maxv)
+{
+    if (want_maxv) BX_HIP(hipMemsetAsync(&c->dsc->maxv_bits, 0, sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_velocity, dim3(blocks_for(c->n, kThreads)), dim3(kThreads), 0, c->stream, c->n,
+                       c->cons, c->u, c->prm.vlimit, c->prm.small_eps,
+                       want_maxv ? &c->dsc->maxv_bits : (unsigned long long *)nullptr);
+    BX_HIP(hipGetLastError());
+    return BX_OK;
+}
+
+int launch_courant(bx_ctx *c)
+{
+    BX_HIP(hipMemsetAsync(&c->dsc->rate_bits[0], 0, sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_courant, dim3(blocks_for(c->n, kThreads)), dim3(kThreads), 0, c->stream, c->n, c->cons,
+                       c->u, c->p, c->adindx, c->prm.dx, c->prm.dy, c->prm.dz, &c->dsc->rate_bits[0]);
+    BX_HIP(hipGetLastError());
+    return BX_OK;
+}
+
+int launch_ghosts(bx_ctx *c, double *planes5)
+{
+    if (c->nghost == 0) return BX_OK;
+    if (c->nscratch > 0) {
+        hipLaunchKernelGGL(k_ghost_snapshot, dim3(blocks_for(c->nscratch, kThreads)), dim3(kThreads), 0, c->stream,
+                           c->n, c->nscratch, c->ghost_scratch_src, planes5, c->ghost_scratch);
+        BX_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_ghosts, dim3(blocks_for(c->nghost, kThreads)), dim3(kThreads), 0, c->stream, c->n,
+                       c->nghost, c->ghost_dst, c->ghost_src, c->ghost_sign, planes5, c->ghost_scratch,
+                       c->nscratch);
+    BX_HIP(hipGetLastError());
+    return BX_OK;
+}
+
+int launch_source(bx_ctx *c, double dt)
+{
+    hipLaunchKernelGGL(k_source, dim3(blocks_for(c->n, kThreads)), dim3(kThreads), 0, c->stream, c->n, c->nx,
+                       c->ny, c->nz, c->cons, c->u, dt, c->grav[0], c->grav[1], c->grav[2], c->prm.small_eps);
+    BX_HIP(hipGetLastError());
+    return launch_ghosts(c, c->cons);
+}
+
+int launch_flux(bx_ctx *c, int avg, double dt)
+{
+    const double dx = c->prm.dx, dy = c->prm.dy, dz = c->prm.dz;
+    hipLaunchKernelGGL(k_flux_states, dim3(blocks_for(c->n, kThreads)), dim3(kThreads), 0, c->stream, c->n,
+                       c->nx, c->ny, c->nz, c->flatz, c->cons, c->u, c->p, c->adindx, c->states, dt, dx, dy, dz,
+                       c->prm.small_eps);
+    BX_HIP(hipGetLastError());
+    const double areaxy = dy * dx, areayz = dy * dz, areaxz = dx * dz, vol = dx * dy * dz;
+    hipLaunchKernelGGL(k_flux_faces, dim3(blocks_for(c->n, kThreads)), dim3(kThreads), 0, c->stream, c->n,
+                       c->nx, c->ny, c->nz, c->flatz, c->states, c->cons, c->cons_old, avg, dt, areayz, areaxz,
+                       areaxy, vol, c->prm.small_rho, c->prm.small_eps);
+    BX_HIP(hipGetLastError());
+    return launch_ghosts(c, c->cons);
+}
+
+int launch_avisc(bx_ctx *c)
+{
+    hipLaunchKernelGGL(k_avisc, dim3(blocks_for(c->n, kThreads)), dim3(kThreads), 0, c->stream, c->n, c->nx,
+                       c->ny, c->nz, c->flatz, c->u, c->qq, c->prm.avmag[0], c->prm.avmag[1], c->prm.avmag[2]);
+    BX_HIP(hipGetLastError());
+    return BX_OK;
+}
+
+}  // namespace bx

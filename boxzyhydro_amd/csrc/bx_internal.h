@@ -1,0 +1,111 @@
+// bx_internal.h -- context layout and launch helpers shared by the HIP sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "boxzy.h"
+
+namespace bx {
+
+// Scalars that live on the device so whole steps can run without host syncs.
+struct DevScalars {
+    unsigned long long rate_bits[2];  // Courant max-rate (non-negative doubles as bits), ping-pong
+    unsigned long long maxv_bits;     // velocity.f90 maxv diagnostic
+    unsigned int flags;               // irregular-branch flags raised by the fused kernels
+    unsigned int pad;
+    double dt;                        // full dt of the current step (main.f90:342)
+    double dtold;                     // main.f90:271
+    double time;                      // main.f90:545
+    double endtime;
+    int step;                         // main.f90:213
+    int active;                       // 1 while the step loop runs (time<endtime)
+};
+
+// flags bits
+enum : unsigned int {
+    FLAG_LOWRHO = 1u,  // a state() call met rho<1e-4 (state.f90:52-56)
+    FLAG_CLAMP = 2u,   // a velocity() call met |u_i|>vlimit (velocity.f90:33)
+};
+
+}  // namespace bx
+
+struct bx_ctx {
+    bx_params prm;
+    int device;
+    hipStream_t stream;
+    int nx, ny, nz;
+    int64_t n;        // ngrid
+    int64_t nghost;
+    double muc, rgas;
+    double grav[3];   // uniform gravity acceleration; -0.0 from phi=0 (utils.f90:566-568)
+    bool flatz;       // nz<6: z neighbours are the cell itself (init_grid.f90:241-247)
+
+    // exact-path device state, SoA planes of n doubles
+    double *cons;      // 5 planes: rho, mx, my, mz, E
+    double *cons_old;  // 5 planes
+    double *u;         // 3 planes
+    double *p;
+    double *adindx;
+    double *qq;        // 3 planes
+    double *states;    // 42 planes: per direction 7 minus + 7 plus interface states
+
+    // ghost table: ghost cell id, donor (>=0: cell id; <0: -(scratch slot+1)),
+    // sign bits (bit k flips momentum k); scratch holds pre-fill values of
+    // ghost donors (5 planes of nscratch)
+    int32_t *ghost_dst;
+    int32_t *ghost_src;
+    int32_t *ghost_sign;
+    int32_t *ghost_scratch_src;
+    double *ghost_scratch;
+    int64_t nscratch;
+
+    // fused-path state
+    double *ustar;     // 5 planes: stage-1 result U*
+    double *pstar;     // stage-2 pressure p* (stale-velocity EOS, main.f90:351-355)
+    double *unext;     // 5 planes: stage-2 result U^{n+1}
+    bool fused_valid;  // device cons holds a regular state (see bx_step)
+    bool aux_stale;    // u, p planes not materialised since the last fused step
+    bool old_stale;    // cons_old not materialised since the last fused step
+    int64_t fallbacks;
+
+    bx::DevScalars *dsc;   // device scalars
+    bx::DevScalars *hsc;   // pinned host mirror
+
+    // mixed-precision planes (FP32 storage)
+    float *cons32, *ustar32, *unext32;
+};
+
+namespace bx {
+
+int set_error(int code, const std::string &msg);
+int hip_check(hipError_t e, const char *what);
+
+#define BX_HIP(call)                                             \
+    do {                                                         \
+        hipError_t _e = (call);                                  \
+        if (_e != hipSuccess) return bx::hip_check(_e, #call);   \
+    } while (0)
+
+inline unsigned int blocks_for(int64_t n, int threads)
+{
+    return (unsigned int)((n + threads - 1) / threads);
+}
+
+// kernels launched by the C-ABI (defined in bx_exact.hip / bx_fused.hip)
+int launch_state(bx_ctx *c);
+int launch_velocity(bx_ctx *c, bool want_maxv);
+int launch_courant(bx_ctx *c);
+int launch_source(bx_ctx *c, double dt);
+int launch_ghosts(bx_ctx *c, double *planes5);
+int launch_flux(bx_ctx *c, int avg, double dt);
+int launch_avisc(bx_ctx *c);
+int launch_materialize(bx_ctx *c);
+int launch_regular_check(bx_ctx *c);
+int fused_step(bx_ctx *c);
+int fused_prepare(bx_ctx *c);
+
+}  // namespace bx

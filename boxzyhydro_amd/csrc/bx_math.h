@@ -1,0 +1,103 @@
+// bx_math.h -- device arithmetic of the hydro step, written to reproduce the
+// reference's FP64 evaluation order exactly.  Build with -ffp-contract=off and
+// without fast-math: every +, *, / and sqrt below is one IEEE-754 operation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#ifndef BX_THETA
+#define BX_THETA 1.25  // generalized-minmod theta: Makefile:20 -DSLOPE_THETA=1.25
+#endif
+
+namespace bx {
+
+template <typename T>
+__device__ __forceinline__ T vmax(T a, T b) { return a > b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T vmin(T a, T b) { return a < b ? a : b; }
+
+// utils.f90:696-707
+template <typename T>
+__device__ __forceinline__ T genminmod(T x1, T x2, T x3)
+{
+    if ((x1 > T(0)) && ((x2 > T(0)) && (x3 > T(0)))) {
+        T tmp = vmin(x2, x3);
+        return vmin(x1, tmp);
+    } else if ((x1 < T(0)) && ((x2 < T(0)) && (x3 < T(0)))) {
+        T tmp = vmax(x2, x3);
+        return vmax(x1, tmp);
+    }
+    return T(0);
+}
+
+// utils.f90:738-741: slope(c0,c1,c2) = genminmod(tt*(c1-c0), half*(c2-c0), tt*(c2-c1))
+template <typename T>
+__device__ __forceinline__ T slope(T c0, T c1, T c2)
+{
+    const T tt = T(BX_THETA);
+    return genminmod(tt * (c1 - c0), T(0.5) * (c2 - c0), tt * (c2 - c1));
+}
+
+// Primitive variables of one cell as the flux routine sees them.
+template <typename T>
+struct Prim {
+    T d, u[3], p, e;  // rho, velocity, pressure, internal energy (eps)
+};
+
+// One directional interface state (flux.f90:336-351): rho, velocity, p, eps, gamma.
+template <typename T>
+struct Side {
+    T d, v[3], p, e, g;
+};
+
+// flux.f90:299-308 interface_muscl
+template <typename T>
+__device__ __forceinline__ void muscl(T q, T lm, T lp, T s, T &minus, T &plus)
+{
+    minus = q + T(0.5) * (T(-1) - lm) * s;
+    plus = q + T(0.5) * (T(1) - lp) * s;
+}
+
+// flux.f90:512-590 for one face: L = plus state of the cell below the face,
+// R = minus state of the cell above it, direction dir (0,1,2).
+// Accumulator type A for the flux itself (double for FP64 and mixed forms).
+template <typename T, int dir>
+__device__ __forceinline__ void face_flux(const Side<T> &L, const Side<T> &R, T F[5])
+{
+    const T h = T(0.5);
+    T sl[5], sr[5], fl[5], fr[5];
+    sl[0] = L.d;
+    sl[1] = L.d * L.v[0];
+    sl[2] = L.d * L.v[1];
+    sl[3] = L.d * L.v[2];
+    sl[4] = L.e + h * L.d * (L.v[0] * L.v[0] + L.v[1] * L.v[1] + L.v[2] * L.v[2]);
+    sr[0] = R.d;
+    sr[1] = R.d * R.v[0];
+    sr[2] = R.d * R.v[1];
+    sr[3] = R.d * R.v[2];
+    sr[4] = R.e + h * R.d * (R.v[0] * R.v[0] + R.v[1] * R.v[1] + R.v[2] * R.v[2]);
+    const T vl = L.v[dir], vr = R.v[dir];
+    const T zero = T(0);
+    // fluxes_l = 0; fluxes_l(2+dir) = pl; then fluxes_l(k) = fluxes_l(k) + states_l(k)*vl
+    fl[0] = sl[0] * vl;
+    fl[1] = (dir == 0 ? L.p : zero) + sl[1] * vl;
+    fl[2] = (dir == 1 ? L.p : zero) + sl[2] * vl;
+    fl[3] = (dir == 2 ? L.p : zero) + sl[3] * vl;
+    fl[4] = (sl[4] + L.p) * vl;
+    fr[0] = sr[0] * vr;
+    fr[1] = (dir == 0 ? R.p : zero) + sr[1] * vr;
+    fr[2] = (dir == 1 ? R.p : zero) + sr[2] * vr;
+    fr[3] = (dir == 2 ? R.p : zero) + sr[3] * vr;
+    fr[4] = (sr[4] + R.p) * vr;
+    // flux_hll, flux.f90:583-589
+    const T cr = sqrt(R.g * R.p / R.d);
+    const T cl = sqrt(L.g * L.p / L.d);
+    const T ap = vmax(vmax(zero, vr + cr), vl + cl);
+    const T am = vmax(vmax(zero, -(vr - cr)), -(vl - cl));
+    const T apam = ap * am;
+    const T den = ap + am;
+#pragma unroll
+    for (int i = 0; i < 5; i++) F[i] = (ap * fl[i] + am * fr[i] - apam * (sr[i] - sl[i])) / den;
+}
+
+}  // namespace bx

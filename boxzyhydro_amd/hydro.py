@@ -1,0 +1,209 @@
+"""Host-side mirror of the reference's hydro-step interface on the GPU.
+
+The reference's hot path is a set of argument-less Fortran subroutines acting
+on module globals (grid_commons.f90:7-36) and called from main.f90:211-555.
+``HydroGrid`` keeps that shape: it owns the device copy of those globals
+(cons, cons_old, u, p, adindx, qq) and the scalars ``dt`` and ``time``, and
+exposes one method per subroutine with the reference's name and argument
+meaning:
+
+=====================  ==========================  ==========================
+method                 reference                   C-ABI entry point
+=====================  ==========================  ==========================
+``state()``            state.f90:6-89              bx_state
+``velocity()``         velocity.f90:4-64           bx_velocity
+``courant()``          courant.f90:4-39 (sets dt)  bx_courant
+``source(tphase,it)``  source.f90:5-205 (uses dt)  bx_source
+``set_ghost_cells()``  utils.f90:305-480           bx_set_ghost_cells
+``flux(avg)``          flux.f90:6-235 (uses dt)    bx_flux
+``avisc()``            avisc.f90:4-26              bx_avisc
+``copy_old()``         main.f90:327-333            bx_copy_old
+``run(...)``           main.f90:196-555            bx_step
+=====================  ==========================  ==========================
+
+Host arrays are the reference's Fortran layout: ``cons`` (ngrid, 5) etc.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+
+from . import native as nv
+from .params import HydroParams
+
+
+def make_params(prm: HydroParams, precision: int = nv.BX_PREC_FP64) -> nv.BxParams:
+    P = nv.BxParams()
+    P.nx, P.ny, P.nz = prm.nx, prm.ny, prm.nz
+    P.dx, P.dy, P.dz = prm.dx, prm.dy, prm.dz
+    P.gammafix = prm.gammafix
+    P.small_rho, P.small_eps, P.vlimit, P.cfl = prm.small_rho, prm.small_eps, prm.vlimit, prm.cfl
+    P.xabun, P.yabun, P.zabun, P.mu_z = prm.xabun, prm.yabun, prm.zabun, prm.mu_z
+    for i, v in enumerate((prm.avmagx, prm.avmagy, prm.avmagz)):
+        P.avmag[i] = v
+    for i, v in enumerate((prm.no_outflow_xl, prm.no_outflow_xr, prm.no_outflow_yl,
+                           prm.no_outflow_yr, prm.no_outflow_zl, prm.no_outflow_zr)):
+        P.no_outflow[i] = int(bool(v))
+    P.precision = precision
+    return P
+
+
+class HydroGrid:
+    """Device-resident state of one grid plus the reference's step scalars."""
+
+    def __init__(self, prm: HydroParams, ic: Optional[Dict[str, np.ndarray]] = None,
+                 device: int = 0, precision: int = nv.BX_PREC_FP64):
+        self.prm = prm
+        self._h = ctypes.c_void_p()
+        nv.check(nv.lib().bx_create(ctypes.byref(make_params(prm, precision)), device,
+                                    ctypes.byref(self._h)), "bx_create")
+        ng = ctypes.c_int64()
+        gh = ctypes.c_int64()
+        nv.check(nv.lib().bx_ngrid(self._h, ctypes.byref(ng), ctypes.byref(gh)), "bx_ngrid")
+        self.ngrid, self.nghost = ng.value, gh.value
+        # main.f90:194-203 driver scalars
+        self.dt = 0.0
+        self.time = prm.starttime
+        self.dtold = 0.0
+        self.step_count = 0
+        if ic is not None:
+            self.upload(ic)
+
+    def close(self) -> None:
+        if self._h:
+            nv.lib().bx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ transfers
+    def upload(self, ic: Dict[str, np.ndarray]) -> None:
+        n = self.ngrid
+        arrs = []
+        for key, k in (("cons", 5), ("u", 3), ("p", 1), ("adindx", 1)):
+            a = ic.get(key)
+            if a is not None:
+                a = np.ascontiguousarray(a, np.float64)
+                if a.size != n * k:
+                    raise ValueError(f"{key}: expected {n * k} values, got {a.size}")
+            arrs.append(a)
+        nv.check(nv.lib().bx_upload(self._h, *[nv.dptr(a) for a in arrs]), "bx_upload")
+
+    def download(self) -> Dict[str, np.ndarray]:
+        n = self.ngrid
+        out = {"cons": np.empty((n, 5)), "u": np.empty((n, 3)), "p": np.empty(n), "adindx": np.empty(n)}
+        nv.check(nv.lib().bx_download(self._h, nv.dptr(out["cons"]), nv.dptr(out["u"]),
+                                      nv.dptr(out["p"]), nv.dptr(out["adindx"])), "bx_download")
+        return out
+
+    def download_cons_old(self) -> np.ndarray:
+        out = np.empty((self.ngrid, 5))
+        nv.check(nv.lib().bx_download_cons_old(self._h, nv.dptr(out)), "bx_download_cons_old")
+        return out
+
+    # ------------------------------------------------ reference subroutines
+    def state(self) -> None:
+        nv.check(nv.lib().bx_state(self._h), "bx_state")
+
+    def velocity(self) -> float:
+        maxv = ctypes.c_double()
+        nv.check(nv.lib().bx_velocity(self._h, ctypes.byref(maxv)), "bx_velocity")
+        return maxv.value
+
+    def courant(self) -> float:
+        dt = ctypes.c_double()
+        nv.check(nv.lib().bx_courant(self._h, ctypes.byref(dt)), "bx_courant")
+        self.dt = dt.value
+        return self.dt
+
+    def source(self, tphase: float, siter: int) -> None:
+        nv.check(nv.lib().bx_source(self._h, tphase, siter, self.dt), "bx_source")
+
+    def set_ghost_cells(self) -> None:
+        nv.check(nv.lib().bx_set_ghost_cells(self._h), "bx_set_ghost_cells")
+
+    def flux(self, avg: int) -> None:
+        nv.check(nv.lib().bx_flux(self._h, avg, self.dt), "bx_flux")
+
+    def copy_old(self) -> None:
+        nv.check(nv.lib().bx_copy_old(self._h), "bx_copy_old")
+
+    def avisc(self) -> np.ndarray:
+        qq = np.empty((self.ngrid, 3))
+        nv.check(nv.lib().bx_avisc(self._h, nv.dptr(qq)), "bx_avisc")
+        return qq
+
+    def sync(self) -> None:
+        nv.check(nv.lib().bx_sync(self._h), "bx_sync")
+
+    # ------------------------------------------------------------ main loop
+    def run(self, nsteps: int, endtime: Optional[float] = None, mode: int = nv.BX_STEP_FUSED,
+            record_dt: bool = True) -> np.ndarray:
+        """Advance up to ``nsteps`` steps of main.f90:211-555 (or to endtime).
+
+        Returns the full dt of every step taken."""
+        if endtime is None:
+            endtime = self.prm.endtime
+        t = ctypes.c_double(self.time)
+        dto = ctypes.c_double(self.dtold)
+        st = ctypes.c_int32(self.step_count)
+        taken = ctypes.c_int32(0)
+        dts = np.zeros(max(nsteps, 1)) if record_dt else None
+        nv.check(nv.lib().bx_step(self._h, mode, nsteps, endtime, ctypes.byref(t), ctypes.byref(dto),
+                                  ctypes.byref(st), nv.dptr(dts), ctypes.byref(taken)), "bx_step")
+        self.time, self.dtold, self.step_count = t.value, dto.value, st.value
+        return dts[:taken.value].copy() if record_dt else np.zeros(0)
+
+    def fallback_count(self) -> int:
+        c = ctypes.c_int64()
+        nv.check(nv.lib().bx_fallback_count(self._h, ctypes.byref(c)), "bx_fallback_count")
+        return c.value
+
+
+def step_sequence(grid, nsteps: int, endtime: float) -> Tuple[list, list]:
+    """The main.f90:211-555 call sequence written out through the per-subroutine
+    methods -- what a driver keeping the reference's calls does.  Works on any
+    object with the reference's subroutine names (HydroGrid or the oracle)."""
+    dts, times = [], []
+    f1 = 1.0
+    while grid.time < endtime and grid.step_count < nsteps:
+        grid.step_count += 1
+        grid.state()
+        dt = grid.courant()
+        dt = dt * f1
+        if grid.step_count > 1 and grid.dtold > 0.0:
+            dt = min(dt, grid.dtold * 1.1)
+        grid.dtold = dt
+        if dt + grid.time > endtime:
+            dt = endtime - grid.time
+        grid.dt = dt * 0.5
+        grid.source(grid.time, 0)
+        grid.set_ghost_cells()
+        grid.state()
+        grid.velocity()
+        grid.copy_old()
+        grid.dt = grid.dt * 2.0
+        grid.flux(0)
+        grid.set_ghost_cells()
+        grid.state()
+        grid.velocity()
+        grid.flux(1)
+        grid.set_ghost_cells()
+        grid.state()
+        grid.velocity()
+        grid.dt = 0.5 * grid.dt
+        grid.source(grid.time + grid.dt, 1)
+        grid.set_ghost_cells()
+        grid.state()
+        grid.velocity()
+        grid.dt = grid.dt * 2.0
+        grid.time = grid.time + grid.dt
+        dts.append(grid.dt)
+        times.append(grid.time)
+    return dts, times

@@ -1,0 +1,102 @@
+"""ctypes binding of libboxzy_hip.so (the C-ABI declared in include/boxzy.h).
+
+The library is built in-tree (``python -m boxzyhydro_amd.build`` or
+``__graft_entry__.build()``).  Loading fails loudly when it is missing: there
+is no CPU fallback on the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libboxzy_hip.so")
+
+BX_OK = 0
+BX_PREC_FP64 = 0
+BX_PREC_MIXED = 1
+BX_STEP_EXACT = 0
+BX_STEP_FUSED = 1
+
+
+class BxParams(ctypes.Structure):
+    """Mirror of ``bx_params`` (include/boxzy.h)."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+        ("gammafix", ctypes.c_double),
+        ("small_rho", ctypes.c_double), ("small_eps", ctypes.c_double),
+        ("vlimit", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("xabun", ctypes.c_double), ("yabun", ctypes.c_double),
+        ("zabun", ctypes.c_double), ("mu_z", ctypes.c_double),
+        ("avmag", ctypes.c_double * 3),
+        ("no_outflow", ctypes.c_int32 * 6),
+        ("precision", ctypes.c_int32),
+    ]
+
+
+class BoxzyError(RuntimeError):
+    pass
+
+
+_lib = None
+
+# name -> argtypes; every entry point returns int
+_VP = ctypes.c_void_p
+_DP = ctypes.POINTER(ctypes.c_double)
+_I32 = ctypes.c_int32
+_SIGS = {
+    "bx_create": [ctypes.POINTER(BxParams), _I32, ctypes.POINTER(_VP)],
+    "bx_destroy": [_VP],
+    "bx_ngrid": [_VP, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
+    "bx_ghost_table": [ctypes.POINTER(BxParams), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                       ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32), ctypes.POINTER(_I32)],
+    "bx_upload": [_VP, _DP, _DP, _DP, _DP],
+    "bx_download": [_VP, _DP, _DP, _DP, _DP],
+    "bx_download_cons_old": [_VP, _DP],
+    "bx_state": [_VP],
+    "bx_velocity": [_VP, _DP],
+    "bx_courant": [_VP, _DP],
+    "bx_source": [_VP, ctypes.c_double, _I32, ctypes.c_double],
+    "bx_set_ghost_cells": [_VP],
+    "bx_flux": [_VP, _I32, ctypes.c_double],
+    "bx_copy_old": [_VP],
+    "bx_avisc": [_VP, _DP],
+    "bx_step": [_VP, _I32, _I32, ctypes.c_double, _DP, _DP, ctypes.POINTER(_I32), _DP,
+                ctypes.POINTER(_I32)],
+    "bx_fallback_count": [_VP, ctypes.POINTER(ctypes.c_int64)],
+    "bx_sync": [_VP],
+}
+
+EXPORTED = tuple(_SIGS) + ("bx_last_error",)
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BoxzyError(f"{LIB_PATH} is missing: build it with `python -m boxzyhydro_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        L.bx_last_error.argtypes = []
+        L.bx_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != BX_OK:
+        msg = lib().bx_last_error().decode(errors="replace")
+        raise BoxzyError(f"{what} failed ({rc}): {msg}")
+
+
+def dptr(a):
+    """double* of a contiguous float64 numpy array, or NULL for None."""
+    if a is None:
+        return ctypes.cast(None, _DP)
+    return a.ctypes.data_as(_DP)

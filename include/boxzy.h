@@ -1,0 +1,112 @@
+/*
+ * boxzy.h -- C-ABI of libboxzy_hip.so, the MI355X (gfx950) implementation of
+ * the BOXZYHYDRO hydro step.
+ *
+ * The reference has no plugin or FFI layer: its hot path is a set of
+ * argument-less Fortran subroutines acting on module globals
+ * (grid_commons.f90:7-36) and called by main.f90.  Each entry point below
+ * replaces one of those subroutines; the Fortran driver binds them with
+ * ISO_C_BINDING and the Python package binds them with ctypes.
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a negative code on failure;
+ *    bx_last_error() returns the message of the last failure on the calling
+ *    thread.  Nothing aborts.
+ *  - Host arrays use the reference's Fortran layout: cons(5,ngrid),
+ *    u(3,ngrid), p(ngrid), adindx(ngrid), qq(3,ngrid), cell id
+ *    ix+(iy-1)nx+(iz-1)nx*ny.  Device state is SoA planes owned by the context.
+ *  - A context is used by one host thread at a time; all work is ordered on
+ *    the context's stream.  Entry points that return a host value
+ *    synchronise that stream.
+ */
+#ifndef BOXZY_H
+#define BOXZY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BX_OK 0
+#define BX_ERR_ARG (-1)
+#define BX_ERR_HIP (-2)
+#define BX_ERR_STATE (-3)
+#define BX_ERR_UNSUPPORTED (-4)
+
+#define BX_PREC_FP64 0  /* plain FP64 drop-in */
+#define BX_PREC_MIXED 1 /* FP32 storage/reconstruction, FP64 update and pressure recovery */
+
+/* Namelist values that reach the hydro step (input.f90:7-16, parameters.f90). */
+typedef struct bx_params {
+    int32_t nx, ny, nz;       /* &grid_params: cells per axis incl. 2 ghost layers per side */
+    double dx, dy, dz;        /* &grid_params */
+    double gammafix;          /* &eos_input (H2STAT=-1) */
+    double small_rho, small_eps, vlimit, cfl; /* &hydro */
+    double xabun, yabun, zabun, mu_z;         /* &eos_input, enter muc (eos.f90:144) */
+    double avmag[3];          /* &hydro avmagx, avmagy, avmagz */
+    int32_t no_outflow[6];    /* &grid_params no_outflow_{xl,xr,yl,yr,zl,zr} */
+    int32_t precision;        /* BX_PREC_FP64 or BX_PREC_MIXED */
+} bx_params;
+
+typedef struct bx_ctx bx_ctx;
+
+const char *bx_last_error(void);
+
+/* Replaces init_grid.f90:7-265 (allocation, boundary flags, neighbour and
+ * ghost-donor tables) for the device state.  Selects device `device`. */
+int bx_create(const bx_params *prm, int32_t device, bx_ctx **out);
+int bx_destroy(bx_ctx *ctx);
+int bx_ngrid(const bx_ctx *ctx, int64_t *ngrid, int64_t *nghost);
+
+/* Host-only: the gather table that one set_ghost_cells call (utils.f90:305-480)
+ * reduces to for this grid and wall set.  Ghost dst[k] receives cell src[k]
+ * (src<0: slot -src-1 of scratch, i.e. the pre-call value of ghost
+ * scratch_src[slot]) with momentum component j negated where bit j of sign[k]
+ * is set.  Pass NULL arrays to query the sizes first.  Needs no GPU. */
+int bx_ghost_table(const bx_params *prm, int64_t *nghost, int64_t *nscratch, int32_t *dst, int32_t *src,
+                   int32_t *sign, int32_t *scratch_src);
+
+/* Host <-> device copies of the reference's global arrays (Fortran layout).
+ * Any pointer may be NULL to skip that array.  upload also sets
+ * cons_old = cons, as init_conditions.f90:69-77 does. */
+int bx_upload(bx_ctx *ctx, const double *cons, const double *u, const double *p, const double *adindx);
+int bx_download(bx_ctx *ctx, double *cons, double *u, double *p, double *adindx);
+int bx_download_cons_old(bx_ctx *ctx, double *cons_old);
+
+/* ---- one entry point per reference subroutine on the path ---- */
+int bx_state(bx_ctx *ctx);                        /* state.f90:6-89 (H2STAT=-1)         */
+int bx_velocity(bx_ctx *ctx, double *maxv);       /* velocity.f90:4-64; maxv may be NULL */
+int bx_courant(bx_ctx *ctx, double *dt);          /* courant.f90:4-39: dt = cfl/max rate */
+int bx_source(bx_ctx *ctx, double tphase, int32_t siter, double dt); /* source.f90:5-205 */
+int bx_set_ghost_cells(bx_ctx *ctx);              /* utils.f90:305-480                   */
+int bx_flux(bx_ctx *ctx, int32_t avg, double dt); /* flux.f90:6-235                      */
+int bx_copy_old(bx_ctx *ctx);                     /* main.f90:327-333 cons_old = cons    */
+int bx_avisc(bx_ctx *ctx, double *qq);            /* avisc.f90:4-26; qq(3,ngrid) host or NULL */
+
+/* ---- whole steps: main.f90:211-555 ----
+ * Advances up to `nsteps` steps or until *time >= endtime.  *time, *dtold and
+ * *step carry the driver's scalars between calls (main.f90:196-198: start
+ * with dtold=0, step=0).  dts (may be NULL) receives each step's full dt.
+ * mode BX_STEP_EXACT runs the reference call sequence kernel by kernel;
+ * BX_STEP_FUSED runs two fused stencil kernels per step and falls back to the
+ * exact sequence for any step that meets a low-density cell (rho<1e-4), the
+ * speed clamp, or a first step whose ghosts are not yet filled.  Both give
+ * results bit-identical to the reference in FP64.
+ * Returns the number of steps taken in *taken. */
+#define BX_STEP_EXACT 0
+#define BX_STEP_FUSED 1
+int bx_step(bx_ctx *ctx, int32_t mode, int32_t nsteps, double endtime, double *time, double *dtold,
+            int32_t *step, double *dts, int32_t *taken);
+
+/* Number of steps that bx_step(BX_STEP_FUSED) had to redo with the exact
+ * sequence since the context was created. */
+int bx_fallback_count(const bx_ctx *ctx, int64_t *count);
+
+/* Synchronise the context's stream. */
+int bx_sync(bx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

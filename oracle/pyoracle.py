@@ -170,7 +170,7 @@ def run_reference(prm, ic: Dict[str, np.ndarray], maxsteps: int, mode: int = 0,
     """Run the reference's own subroutines (oracle/_ref/ref_harness) on ``ic``.
 
     Returns dt/time per step and the final cons/u/p; with mode=1 also the
-    snapshots after every sub-call of step 1 and avisc's qq."""
+    snapshots after every sub-call of step 1."""
     if not have_reference():
         raise RuntimeError("reference driver not built (make -C oracle ref)")
     from boxzyhydro_amd.ics import write_ic_file
@@ -203,5 +203,4 @@ def run_reference(prm, ic: Dict[str, np.ndarray], maxsteps: int, mode: int = 0,
                               "p": arr[1 + 8 * n:1 + 9 * n].copy(),
                               "cons_old": arr[1 + 9 * n:1 + 14 * n].reshape(n, 5).copy()})
             res["snaps"] = snaps
-            res["qq"] = np.fromfile(os.path.join(td, "out.qq"), np.float64).reshape(n, 3).copy()
         return res

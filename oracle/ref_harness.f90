@@ -2,7 +2,7 @@
 ! Test-infrastructure driver for the reference BOXZYHYDRO hydro step.
 !
 ! This program is part of the oracle: it links the reference's own Fortran
-! subroutines (state, velocity, courant, source, flux, set_ghost_cells, avisc,
+! subroutines (state, velocity, courant, source, flux, set_ghost_cells,
 ! init_grid, the eos and units setup) compiled from their sources, and runs the
 ! main-loop body of main.f90:211-555 with every optional-physics flag off.
 ! It is never shipped and never called by the product path; its outputs are
@@ -14,9 +14,12 @@
 !   mode 0    run steps; write <prefix>.dt (dt, time per step) and
 !             <prefix>.final (cons, u, p after the last step)
 !   mode 1    run steps; additionally write <prefix>.snap with the state after
-!             every sub-call of the first step (tagged records) and <prefix>.qq
-!             (avisc output on the initial velocities)
+!             every sub-call of the first step (tagged records)
 !   diag 1    also run the per-step conservation diagnostics (main.f90:222-223)
+!
+! avisc (avisc.f90:4-26) is not driven here: for outer-layer ghosts it reads
+! grid(0) and u(:,0) (ineigh(2:6)=0 from init_grid.f90:178), outside the
+! arrays, so its output there is undefined and the read can fault.
 !
 ! The step sequence below restates main.f90:211-555 with RADTRAN, COOL,
 ! SELFGRAVITY, PARTICLE, EXTERNALGRAV, ENERGYCOR off and fluxangmom=.false.
@@ -71,13 +74,6 @@ program ref_harness
  if(mode==1)then
    open(unit=203,file=trim(prefix)//'.snap',access='stream',form='unformatted',status='replace')
    call snap(0)
-   ! avisc on the loaded velocities (never called by main.f90; avisc.f90:4-26)
-!$OMP PARALLEL DEFAULT(SHARED)
-   call avisc()
-!$OMP END PARALLEL
-   open(unit=204,file=trim(prefix)//'.qq',access='stream',form='unformatted',status='replace')
-   write(204)qq
-   close(204)
  endif
 
  open(unit=202,file=trim(prefix)//'.dt',access='stream',form='unformatted',status='replace')

@@ -1,0 +1,182 @@
+"""Shared helpers of the parity tests: rebuild a golden case's inputs, digest
+arrays, and drive any grid object through the main.f90:211-555 call sequence
+with snapshot hooks (tags match oracle/ref_harness.f90)."""
+from __future__ import annotations
+
+import hashlib
+from typing import Callable, Dict, Optional
+
+import numpy as np
+
+from boxzyhydro_amd import ics
+from boxzyhydro_amd.params import HydroParams
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, "<f8").tobytes()).hexdigest()
+
+
+def ic_digest(ic) -> str:
+    h = hashlib.sha256()
+    for k in ("cons", "u", "p", "adindx"):
+        h.update(np.ascontiguousarray(ic[k], "<f8").tobytes())
+    return h.hexdigest()
+
+
+def stress_ic(prm: HydroParams, seed: int):
+    """Random box with forced branches: low-density cells (state.f90:52-56),
+    speeds above vlimit (velocity.f90:33-50) and internal energy below the
+    kinetic energy (eps floors, flux.f90:226-228)."""
+    ic = ics.box_ic(prm, seed=seed, sigma_cells=1.0, mach_rms=1.0)
+    rng = np.random.Generator(np.random.PCG64(seed + 100))
+    n = prm.ngrid
+    b = ics.boundary_flags(prm).reshape(n)
+    interior = np.flatnonzero(b == 0)
+    pick = rng.choice(interior, size=min(len(interior), 24), replace=False)
+    lowrho, fast, cold = pick[:8], pick[8:16], pick[16:24]
+    cons, u = ic["cons"], ic["u"]
+    cons[lowrho, 0] = 5e-5
+    cons[lowrho, 1:4] = cons[lowrho, 0:1] * u[lowrho]
+    cons[lowrho, 4] = 1e-4
+    u[fast] *= 25.0
+    cons[fast, 1:4] = cons[fast, 0:1] * u[fast]
+    cons[fast, 4] = 1.0 + 0.5 * cons[fast, 0] * np.sum(u[fast] ** 2, axis=1)
+    cons[cold, 4] = 0.25 * 0.5 * cons[cold, 0] * np.sum(u[cold] ** 2, axis=1)
+    return ic
+
+
+def case_inputs(rec: dict):
+    """(params, ic) of a golden case, regenerated from its generator record."""
+    prm = HydroParams(**rec["params"])
+    gen = rec["generator"]
+    kind = gen["kind"]
+    if kind == "sod":
+        ic = ics.sod_ic(prm, axis=gen["axis"], x0=gen["x0"])
+    elif kind == "box":
+        ic = ics.box_ic(prm, seed=gen["seed"], sigma_cells=gen["sigma_cells"])
+    elif kind == "stress":
+        ic = stress_ic(prm, gen["seed"])
+    elif kind == "riemann_batch":
+        batch = ics.riemann_batch(gen["n_problems"], seed=gen["seed"], lmin=gen["lmin"], lmax=gen["lmax"])
+        prm2, ic = batch[gen["index"]]
+        assert prm2 == prm
+    else:
+        raise ValueError(kind)
+    return prm, ic
+
+
+class Driver:
+    """main.f90:211-555 over a grid exposing the reference's subroutines as
+    ``state() courant()->dt source(dt) set_ghost_cells() velocity() copy_old()
+    flux(avg, dt)``.  ``snap(tag, dt)`` is called where the reference driver
+    (oracle/ref_harness.f90) records its step-1 snapshots."""
+
+    def __init__(self, grid, starttime: float = 0.0):
+        self.g = grid
+        self.time = starttime
+        self.dtold = 0.0
+        self.step = 0
+
+    def run(self, maxsteps: int, endtime: float, snap: Optional[Callable[[int, float], None]] = None):
+        g = self.g
+        dts, times = [], []
+        f1 = 1.0
+        while not (self.time >= endtime or self.step >= maxsteps):
+            self.step += 1
+            s = snap if self.step == 1 else None
+            g.state()
+            dt = g.courant()
+            if s:
+                s(1, dt)
+            dt = dt * f1
+            if self.step > 1 and self.dtold > 0.0:
+                dt = min(dt, self.dtold * 1.1)
+            self.dtold = dt
+            if dt + self.time > endtime:
+                dt = endtime - self.time
+            dt = dt * 0.5
+            g.source(dt)
+            if s:
+                s(2, dt)
+            g.set_ghost_cells()
+            g.state()
+            if s:
+                s(3, dt)
+            g.velocity()
+            g.copy_old()
+            if s:
+                s(4, dt)
+            dt = dt * 2.0
+            g.flux(0, dt)
+            if s:
+                s(5, dt)
+            g.set_ghost_cells()
+            g.state()
+            g.velocity()
+            if s:
+                s(6, dt)
+            g.flux(1, dt)
+            if s:
+                s(7, dt)
+            g.set_ghost_cells()
+            g.state()
+            g.velocity()
+            if s:
+                s(8, dt)
+            dt = 0.5 * dt
+            g.source(dt)
+            g.set_ghost_cells()
+            g.state()
+            g.velocity()
+            if s:
+                s(9, dt)
+            dt = dt * 2.0
+            self.time = self.time + dt
+            dts.append(dt)
+            times.append(self.time)
+        return dts, times
+
+
+class OracleAdapter:
+    """OracleGrid with the Driver's method signatures."""
+
+    def __init__(self, og):
+        self.og = og
+
+    def state(self): self.og.state()
+    def courant(self): return self.og.courant()
+    def source(self, dt): self.og.source(dt)
+    def set_ghost_cells(self): self.og.set_ghost_cells()
+    def velocity(self): self.og.velocity()
+    def copy_old(self): self.og.copy_old()
+    def flux(self, avg, dt): self.og.flux(avg, dt)
+
+    def arrays(self) -> Dict[str, np.ndarray]:
+        return {"cons": self.og.cons, "u": self.og.u, "p": self.og.p, "cons_old": self.og.cons_old}
+
+
+class HipAdapter:
+    """HydroGrid with the Driver's method signatures (dt passed explicitly)."""
+
+    def __init__(self, hg):
+        self.hg = hg
+
+    def state(self): self.hg.state()
+    def courant(self): return self.hg.courant()
+
+    def source(self, dt):
+        self.hg.dt = dt
+        self.hg.source(0.0, 0)
+
+    def set_ghost_cells(self): self.hg.set_ghost_cells()
+    def velocity(self): self.hg.velocity()
+    def copy_old(self): self.hg.copy_old()
+
+    def flux(self, avg, dt):
+        self.hg.dt = dt
+        self.hg.flux(avg)
+
+    def arrays(self) -> Dict[str, np.ndarray]:
+        d = self.hg.download()
+        d["cons_old"] = self.hg.download_cons_old()
+        return d

@@ -1,0 +1,27 @@
+"""The C-ABI library loads and exports every symbol include/boxzy.h declares
+(no GPU work is launched)."""
+import os
+import re
+
+from conftest import ROOT
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "boxzy.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(bx_[a-z_0-9]+)\s*\(", text, re.M)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    for s in ("bx_create", "bx_state", "bx_velocity", "bx_courant", "bx_source", "bx_flux",
+              "bx_set_ghost_cells", "bx_avisc", "bx_step", "bx_last_error"):
+        assert s in syms
+
+
+def test_library_exports_all_declared_symbols():
+    from boxzyhydro_amd import native as nv
+
+    L = nv.lib()
+    for s in declared_symbols():
+        assert hasattr(L, s), s
+    assert set(declared_symbols()) <= set(nv.EXPORTED)
