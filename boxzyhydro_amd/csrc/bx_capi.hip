@@ -315,6 +315,8 @@ int bx_destroy(bx_ctx *c)
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (c->hsc) (void)hipHostFree(c->hsc);
+    for (int i = 0; i < 4; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return BX_OK;
@@ -394,6 +396,7 @@ int bx_upload(bx_ctx *c, const double *cons, const double *u, const double *p, c
     c->fused_valid = false;
     c->aux_stale = false;
     c->old_stale = false;
+    if (cons) c->ghosts_filled = false;
     return BX_OK;
 }
 
@@ -587,26 +590,136 @@ static int exact_step(bx_ctx *c, double endtime, double *time, double *dtold, in
 
 }  // namespace bx
 
+namespace bx {
+
+static double rate_from_bits(unsigned long long b)
+{
+    double r;
+    std::memcpy(&r, &b, sizeof(double));
+    return r;
+}
+
+// main.f90:266-281 on the host (same IEEE operations as the reference)
+static double step_dt(double rate, double cfl, double endtime, double time, double *dtold, int32_t step)
+{
+    double dt = cfl / rate;  // courant.f90:29
+    dt = dt * 1.0;
+    if (step > 1 && *dtold > 0.0) dt = std::min(dt, *dtold * 1.1);
+    *dtold = dt;
+    if (dt + time > endtime) dt = endtime - time;
+    dt = dt * 0.5;
+    return dt * 2.0;  // full dt of the flux stages (main.f90:342)
+}
+
+static int read_scalars(bx_ctx *c)
+{
+    BX_HIP(hipMemcpyAsync(c->hsc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+    BX_HIP(hipStreamSynchronize(c->stream));
+    return BX_OK;
+}
+
+// Fused stepping (see include/boxzy.h): returns after `nsteps` steps or endtime.
+static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, double *dtold, int32_t *step,
+                      double *dts, int32_t *taken)
+{
+    int rc;
+    int32_t k = 0;
+    bool have_rate = false;
+    while (k < nsteps && *time < endtime && *step < 1000000) {
+        if (!c->fused_valid) {
+            // the state comes from the exact path or an upload: check it
+            if (c->aux_stale) {
+                if ((rc = launch_materialize(c))) return rc;
+                c->aux_stale = false;
+            }
+            bool ok = c->ghosts_filled;
+            if (ok) {
+                if ((rc = launch_regular_check(c))) return rc;
+                if ((rc = read_scalars(c))) return rc;
+                ok = c->hsc->flags == 0;
+            }
+            if (!ok) {
+                double dt;
+                if ((rc = exact_step(c, endtime, time, dtold, step, &dt))) return rc;
+                c->ghosts_filled = true;
+                if (dts) dts[k] = dt;
+                k++;
+                continue;
+            }
+            c->fused_valid = true;
+            have_rate = true;
+        }
+        if (!have_rate) return set_error(BX_ERR_STATE, "fused step without a Courant rate");
+        const double rate = rate_from_bits(c->hsc->rate_bits[0]);
+        const double time0 = *time, dtold0 = *dtold;
+        const int32_t step0 = *step;
+        *step += 1;
+        const double dt = step_dt(rate, c->prm.cfl, endtime, *time, dtold, *step);
+        if ((rc = fused_step(c, dt))) return rc;
+        if ((rc = read_scalars(c))) return rc;
+        if (c->hsc->flags) {
+            // an irregular branch: redo this step with the exact sequence from U^n
+            c->fallbacks++;
+            *time = time0;
+            *dtold = dtold0;
+            *step = step0;
+            if ((rc = launch_materialize(c))) return rc;
+            c->aux_stale = false;
+            c->fused_valid = false;
+            double dtx;
+            if ((rc = exact_step(c, endtime, time, dtold, step, &dtx))) return rc;
+            if (dts) dts[k] = dtx;
+            k++;
+            continue;
+        }
+        if (c->prof_on) {
+            float t1 = 0.f, t2 = 0.f;
+            BX_HIP(hipEventElapsedTime(&t1, c->ev[0], c->ev[1]));
+            BX_HIP(hipEventElapsedTime(&t2, c->ev[2], c->ev[3]));
+            c->prof_ms[0] += t1;
+            c->prof_ms[1] += t2;
+            c->prof_steps++;
+        }
+        std::swap(c->cons, c->unext);
+        c->aux_stale = true;
+        c->old_stale = true;
+        *time = *time + dt;  // main.f90:544-545 (dt*half*two == dt)
+        if (dts) dts[k] = dt;
+        k++;
+    }
+    *taken = k;
+    return BX_OK;
+}
+
+}  // namespace bx
+
 extern "C" int bx_step(bx_ctx *c, int32_t mode, int32_t nsteps, double endtime, double *time, double *dtold,
                        int32_t *step, double *dts, int32_t *taken)
 {
     if (!c || !time || !dtold || !step) return set_error(BX_ERR_ARG, "bx_step: null argument");
     if (mode != BX_STEP_EXACT && mode != BX_STEP_FUSED) return set_error(BX_ERR_ARG, "bx_step: unknown mode");
     BX_HIP(hipSetDevice(c->device));
+    if (c->prm.precision != BX_PREC_FP64) return set_error(BX_ERR_UNSUPPORTED, "bx_step: precision not supported here");
     int32_t k = 0;
     int rc = BX_OK;
-    if (mode == BX_STEP_EXACT || c->prm.precision != BX_PREC_FP64) {
+    if (mode == BX_STEP_FUSED && !c->fused_ready && !c->fused_unsupported) {
+        rc = fused_prepare(c);
+        if (rc == BX_ERR_UNSUPPORTED) c->fused_unsupported = true;  // deep ghost chains: exact steps
+        else if (rc) return rc;
+        else c->fused_ready = true;
+        rc = BX_OK;
+    }
+    if (mode == BX_STEP_FUSED && c->fused_ready) {
+        rc = fused_loop(c, nsteps, endtime, time, dtold, step, dts, &k);
+    } else {
         if ((rc = ensure_aux(c))) return rc;
-        if (c->prm.precision != BX_PREC_FP64)
-            return set_error(BX_ERR_UNSUPPORTED, "bx_step: the exact sequence is FP64 only");
         while (k < nsteps && *time < endtime && *step < 1000000) {
             double dt;
             if ((rc = exact_step(c, endtime, time, dtold, step, &dt))) break;
+            c->ghosts_filled = true;
             if (dts) dts[k] = dt;
             k++;
         }
-    } else {
-        return set_error(BX_ERR_UNSUPPORTED, "bx_step: fused mode not built yet");
     }
     if (taken) *taken = k;
     if (rc) return rc;
@@ -614,6 +727,31 @@ extern "C" int bx_step(bx_ctx *c, int32_t mode, int32_t nsteps, double endtime, 
     return BX_OK;
 }
 
-namespace bx {
-int launch_materialize(bx_ctx *) { return set_error(BX_ERR_UNSUPPORTED, "materialize: not built yet"); }
-}  // namespace bx
+extern "C" int bx_tune(bx_ctx *c, int32_t row_tile, int32_t z_chunk)
+{
+    if (!c) return set_error(BX_ERR_ARG, "bx_tune: null context");
+    if (row_tile != 4 && row_tile != 8 && row_tile != 12) return set_error(BX_ERR_ARG, "bx_tune: row tile 4, 8 or 12");
+    if (z_chunk < 1) return set_error(BX_ERR_ARG, "bx_tune: z chunk must be >= 1");
+    c->fused_ty = row_tile;
+    c->fused_lz = z_chunk;
+    return BX_OK;
+}
+
+extern "C" int bx_profile(bx_ctx *c, int32_t enable, double *stage_ms, int64_t *steps)
+{
+    if (!c) return set_error(BX_ERR_ARG, "bx_profile: null context");
+    BX_HIP(hipSetDevice(c->device));
+    if (enable && !c->ev[0])
+        for (int i = 0; i < 4; i++) BX_HIP(hipEventCreate(&c->ev[i]));
+    if (stage_ms) {
+        stage_ms[0] = c->prof_ms[0];
+        stage_ms[1] = c->prof_ms[1];
+    }
+    if (steps) *steps = c->prof_steps;
+    if (enable >= 0) {
+        c->prof_on = enable != 0;
+        c->prof_ms[0] = c->prof_ms[1] = 0.0;
+        c->prof_steps = 0;
+    }
+    return BX_OK;
+}

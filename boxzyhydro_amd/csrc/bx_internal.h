@@ -68,6 +68,14 @@ struct bx_ctx {
     double *pstar;     // stage-2 pressure p* (stale-velocity EOS, main.f90:351-355)
     double *unext;     // 5 planes: stage-2 result U^{n+1}
     bool fused_valid;  // device cons holds a regular state (see bx_step)
+    bool fused_ready;  // fused buffers allocated, ghost table shallow
+    bool fused_unsupported;
+    bool ghosts_filled;  // ghost cells hold their donors' values (after any exact step)
+    int fused_ty, fused_lz;  // row tile and z chunk of the stage kernels
+    bool prof_on;
+    hipEvent_t ev[4];
+    double prof_ms[2];
+    int64_t prof_steps;
     bool aux_stale;    // u, p planes not materialised since the last fused step
     bool old_stale;    // cons_old not materialised since the last fused step
     int64_t fallbacks;
@@ -105,7 +113,7 @@ int launch_flux(bx_ctx *c, int avg, double dt);
 int launch_avisc(bx_ctx *c);
 int launch_materialize(bx_ctx *c);
 int launch_regular_check(bx_ctx *c);
-int fused_step(bx_ctx *c);
+int fused_step(bx_ctx *c, double dt);
 int fused_prepare(bx_ctx *c);
 
 }  // namespace bx

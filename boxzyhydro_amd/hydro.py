@@ -160,6 +160,18 @@ class HydroGrid:
         self.time, self.dtold, self.step_count = t.value, dto.value, st.value
         return dts[:taken.value].copy() if record_dt else np.zeros(0)
 
+    def tune(self, row_tile: int, z_chunk: int) -> None:
+        """Tile shape of the fused stage kernels (results are unchanged)."""
+        nv.check(nv.lib().bx_tune(self._h, row_tile, z_chunk), "bx_tune")
+
+    def profile(self, enable: int = -1):
+        """Event timing of the fused stage kernels: enable=1 start, 0 stop,
+        -1 read.  Returns (stage1_ms_total, stage2_ms_total, steps)."""
+        ms = (ctypes.c_double * 2)()
+        steps = ctypes.c_int64()
+        nv.check(nv.lib().bx_profile(self._h, enable, ms, ctypes.byref(steps)), "bx_profile")
+        return ms[0], ms[1], steps.value
+
     def fallback_count(self) -> int:
         c = ctypes.c_int64()
         nv.check(nv.lib().bx_fallback_count(self._h, ctypes.byref(c)), "bx_fallback_count")

@@ -103,6 +103,15 @@ int bx_step(bx_ctx *ctx, int32_t mode, int32_t nsteps, double endtime, double *t
  * sequence since the context was created. */
 int bx_fallback_count(const bx_ctx *ctx, int64_t *count);
 
+/* Tile shape of the fused stage kernels: output rows per block (4, 8 or 12)
+ * and z planes per block.  Used by the autotuner; results do not change. */
+int bx_tune(bx_ctx *ctx, int32_t row_tile, int32_t z_chunk);
+
+/* Event timing of the fused stage kernels on the context's stream.
+ * enable=1 starts (and clears), 0 stops, -1 only reads.  stage_ms[2]
+ * receives the summed durations of stage 1 and stage 2 over *steps steps. */
+int bx_profile(bx_ctx *ctx, int32_t enable, double *stage_ms, int64_t *steps);
+
 /* Synchronise the context's stream. */
 int bx_sync(bx_ctx *ctx);
 
