@@ -624,7 +624,8 @@ static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, d
 {
     int rc;
     int32_t k = 0;
-    bool have_rate = false;
+    // While fused_valid holds, hsc->rate_bits[0] is the Courant rate of the
+    // current state (from the regularity check or the last fused step).
     while (k < nsteps && *time < endtime && *step < 1000000) {
         if (!c->fused_valid) {
             // the state comes from the exact path or an upload: check it
@@ -647,9 +648,7 @@ static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, d
                 continue;
             }
             c->fused_valid = true;
-            have_rate = true;
         }
-        if (!have_rate) return set_error(BX_ERR_STATE, "fused step without a Courant rate");
         const double rate = rate_from_bits(c->hsc->rate_bits[0]);
         const double time0 = *time, dtold0 = *dtold;
         const int32_t step0 = *step;

@@ -732,6 +732,35 @@ void bxo_avisc(bxo_grid *g)
     }
 }
 
+/* ---------------------------------------------- scalar hooks for tests */
+double bxo_slope(double c0, double c1, double c2) { return slope(c0, c1, c2); }
+
+/* one HLL face (flux.f90:512-590): L, R = {rho, vx, vy, vz, p, eps, gamma} */
+void bxo_face(int idir, const double L[7], const double R[7], double F[5])
+{
+    /* reuse get_fluxes through a 2-cell scratch grid of interface states */
+    double fl[5] = {0, 0, 0, 0, 0}, fr[5] = {0, 0, 0, 0, 0};
+    double dl = L[0], vxl = L[1], vyl = L[2], vzl = L[3], pl = L[4], el = L[5], gl = L[6];
+    double dr = R[0], vxr = R[1], vyr = R[2], vzr = R[3], pr = R[4], er = R[5], gr = R[6];
+    double sl[5] = {dl, dl * vxl, dl * vyl, dl * vzl, el + HALF * dl * (vxl * vxl + vyl * vyl + vzl * vzl)};
+    double sr[5] = {dr, dr * vxr, dr * vyr, dr * vzr, er + HALF * dr * (vxr * vxr + vyr * vyr + vzr * vzr)};
+    double vl = idir == 0 ? vxl : (idir == 1 ? vyl : vzl);
+    double vr = idir == 0 ? vxr : (idir == 1 ? vyr : vzr);
+    fl[1 + idir] = pl;
+    fr[1 + idir] = pr;
+    fl[0] = sl[0] * vl;
+    fl[1] = fl[1] + sl[1] * vl;
+    fl[2] = fl[2] + sl[2] * vl;
+    fl[3] = fl[3] + sl[3] * vl;
+    fl[4] = (sl[4] + pl) * vl;
+    fr[0] = sr[0] * vr;
+    fr[1] = fr[1] + sr[1] * vr;
+    fr[2] = fr[2] + sr[2] * vr;
+    fr[3] = fr[3] + sr[3] * vr;
+    fr[4] = (sr[4] + pr) * vr;
+    flux_hll(fl, fr, sl, sr, gl, gr, dl, dr, pl, pr, vl, vr, F);
+}
+
 /* ------------------------------------------------------- diagnostics */
 /* utils.f90:1021-1098 conservation_diagnostic + get_etot (output only) */
 void bxo_diagnostics(bxo_grid *g, double out[12])

@@ -58,6 +58,11 @@ void bxo_diagnostics(bxo_grid *g, double out[12]); /* utils.f90:1021-1098 */
 int bxo_run(bxo_grid *g, int maxsteps, double starttime, double endtime,
             double *dts, double *times, int diag);
 
+/* scalar hooks for closed-form tests: the limiter (utils.f90:738-741) and
+ * one HLL face (flux.f90:512-590), L/R = {rho, vx, vy, vz, p, eps, gamma} */
+double bxo_slope(double c0, double c1, double c2);
+void bxo_face(int idir, const double L[7], const double R[7], double F[5]);
+
 /* set the OpenMP thread count used by the passes (0 = leave default) */
 void bxo_set_threads(int n);
 int bxo_get_threads(void);

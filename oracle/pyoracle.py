@@ -67,6 +67,9 @@ def lib():
         L.bxo_diagnostics.argtypes = [vp, dp]
         L.bxo_run.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp, dp, ctypes.c_int]
         L.bxo_run.restype = ctypes.c_int
+        L.bxo_slope.argtypes = [ctypes.c_double] * 3
+        L.bxo_slope.restype = ctypes.c_double
+        L.bxo_face.argtypes = [ctypes.c_int, dp, dp, dp]
         L.bxo_set_threads.argtypes = [ctypes.c_int]
         L.bxo_get_threads.restype = ctypes.c_int
         _lib = L
@@ -152,6 +155,18 @@ class OracleGrid:
         return dts[:k].copy(), times[:k].copy()
 
 
+def slope(c0: float, c1: float, c2: float) -> float:
+    return lib().bxo_slope(c0, c1, c2)
+
+
+def face(idir: int, L, R) -> np.ndarray:
+    Lv = np.ascontiguousarray(L, np.float64)
+    Rv = np.ascontiguousarray(R, np.float64)
+    F = np.zeros(5)
+    lib().bxo_face(idir, _dp(Lv), _dp(Rv), _dp(F))
+    return F
+
+
 def set_threads(n: int) -> None:
     lib().bxo_set_threads(n)
 
@@ -166,7 +181,7 @@ def have_reference() -> bool:
 
 
 def run_reference(prm, ic: Dict[str, np.ndarray], maxsteps: int, mode: int = 0,
-                  threads: int = 4, diag: int = 0) -> Dict[str, np.ndarray]:
+                  threads: int = 4, diag: int = 0, workdir: Optional[str] = None) -> Dict[str, np.ndarray]:
     """Run the reference's own subroutines (oracle/_ref/ref_harness) on ``ic``.
 
     Returns dt/time per step and the final cons/u/p; with mode=1 also the
@@ -176,7 +191,7 @@ def run_reference(prm, ic: Dict[str, np.ndarray], maxsteps: int, mode: int = 0,
     from boxzyhydro_amd.ics import write_ic_file
 
     n = prm.ngrid
-    with tempfile.TemporaryDirectory() as td:
+    with tempfile.TemporaryDirectory(dir=workdir) as td:
         with open(os.path.join(td, "run.nml"), "w") as f:
             f.write(prm.to_namelist())
         write_ic_file(os.path.join(td, "ic.bin"), ic)
@@ -185,7 +200,9 @@ def run_reference(prm, ic: Dict[str, np.ndarray], maxsteps: int, mode: int = 0,
                        cwd=td, env=env, check=True, stdout=subprocess.DEVNULL)
         dt = np.fromfile(os.path.join(td, "out.dt"), np.float64).reshape(-1, 2)
         fin = np.fromfile(os.path.join(td, "out.final"), np.float64)
+        tim = open(os.path.join(td, "out.timing")).read().split()
         res = {"dt": dt[:, 0].copy(), "time": dt[:, 1].copy(),
+               "wall_s": float(tim[0]), "threads": int(tim[2]),
                "cons": fin[:5 * n].reshape(n, 5).copy(),
                "u": fin[5 * n:8 * n].reshape(n, 3).copy(),
                "p": fin[8 * n:9 * n].copy()}

@@ -11,8 +11,9 @@
 ! Usage:  ref_harness <namelist> <ic_file> <out_prefix> <max_steps> <mode> [diag]
 !   ic_file   stream-binary float64: cons(5,ngrid) u(3,ngrid) p(ngrid) adindx(ngrid)
 !             (Fortran order, exactly the reference's grid_commons arrays)
-!   mode 0    run steps; write <prefix>.dt (dt, time per step) and
-!             <prefix>.final (cons, u, p after the last step)
+!   mode 0    run steps; write <prefix>.dt (dt, time per step),
+!             <prefix>.final (cons, u, p after the last step) and
+!             <prefix>.timing (wall seconds of the step loop, steps, threads)
 !   mode 1    run steps; additionally write <prefix>.snap with the state after
 !             every sub-call of the first step (tagged records)
 !   diag 1    also run the per-step conservation diagnostics (main.f90:222-223)
@@ -31,6 +32,7 @@ program ref_harness
  use eos
  use input
  use utils
+ use omp_lib
  implicit none
 
  character(len=256)::nmlfile,icfile,prefix,arg
@@ -38,6 +40,7 @@ program ref_harness
  integer::maxsteps,mode,diag,step,igrid,idim,ios
  real(pre)::f1,dtold,tphase
  real(pre)::etot_loc0,mass_loc0,f_loc0
+ real(8)::wt0,wt1
  type(units)::scale
 
  call get_command_argument(1,nmlfile)
@@ -83,6 +86,7 @@ program ref_harness
  f1=one
  dtold=zero
  step=0
+ wt0=omp_get_wtime()
 
  do
   if(time>=endtime.or.step>=maxsteps)exit
@@ -174,8 +178,12 @@ program ref_harness
   time=time+dt
   write(202)dt,time
  enddo
+ wt1=omp_get_wtime()
  close(202)
  if(mode==1)close(203)
+ open(unit=206,file=trim(prefix)//'.timing',form='formatted',status='replace')
+ write(206,*)wt1-wt0,step,omp_get_max_threads()
+ close(206)
 
  open(unit=205,file=trim(prefix)//'.final',access='stream',form='unformatted',status='replace')
  write(205)cons

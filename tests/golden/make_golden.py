@@ -112,6 +112,18 @@ def main() -> None:
     rec, _ = case_record(prm, ic, 5, 1, 1, {"kind": "box", "seed": 5, "sigma_cells": 1.0})
     cases["flatz_walls"] = rec
 
+    # the reference's own shipped set-up (in.nml: 8x200x1, walls, Sod along y,
+    # cfl left at its default 0.1, t=0.4), run with its serial ghost order
+    prm = HydroParams(nx=8, ny=200, nz=1, dx=0.01, dy=0.01, dz=0.01, small_rho=1e-12, small_eps=1e-40,
+                      vlimit=10.0, endtime=0.4, dtout=4.0, gammafix=1.4)
+    ic = ics.sod_ic(prm, axis=1)
+    rec, r = case_record(prm, ic, 1_000_000, 1, 0, {"kind": "sod", "axis": 1, "x0": 0.0})
+    col = r["cons"].reshape(prm.nz, prm.ny, prm.nx, 5)[0, :, 3]
+    rec["profile_ix4_iz1"] = {"rho": [float(x) for x in col[:, 0]],
+                              "p": [float(x) for x in r["p"].reshape(prm.nz, prm.ny, prm.nx)[0, :, 3]],
+                              "my": [float(x) for x in col[:, 2]]}
+    cases["readme_sod_y"] = rec
+
     # a sample of the batched Riemann workload (short tubes)
     for i, (prm, ic) in enumerate(ics.riemann_batch(4, seed=0, lmin=64, lmax=256)):
         rec, _ = case_record(prm, ic, 1_000_000, 4, 0,

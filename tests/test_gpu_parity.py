@@ -9,7 +9,7 @@ from cases import Driver, HipAdapter, case_inputs, sha
 pytestmark = pytest.mark.gpu
 
 CASES = ["sod_step1", "box12_outflow", "box12_walls", "box24_outflow", "stress8", "flatz_walls",
-         "riemann4_0", "riemann4_1", "riemann4_2", "riemann4_3", "sod_t02"]
+         "riemann4_0", "riemann4_1", "riemann4_2", "riemann4_3", "sod_t02", "readme_sod_y"]
 
 
 def _grid(prm, ic):
@@ -110,7 +110,7 @@ def test_box64_matches_oracle():
 
 
 @pytest.mark.parametrize("name", ["box24_outflow", "sod_t02", "stress8", "box12_walls", "flatz_walls",
-                                  "riemann4_1", "box12_outflow"])
+                                  "riemann4_1", "box12_outflow", "readme_sod_y"])
 def test_fused_step_matches_reference(golden, name):
     """The fused two-kernel step (with its exact-sequence fallback) gives the
     reference's dt sequence and final state bit for bit."""
@@ -149,3 +149,22 @@ def test_fused_tiles_match_exact_box64(tile):
     for k in ("cons", "u", "p"):
         assert _bits_equal(oe[k], of[k]), k
     assert gf.fallback_count() == 0
+
+
+def test_fused_resumes_across_calls():
+    """Stepping in several bx_step calls equals one call (scalars carried)."""
+    from boxzyhydro_amd import ics
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.params import box_params
+
+    prm = box_params(32)
+    ic = ics.box_ic(prm, seed=5)
+    ga = _grid(prm, ic)
+    da = ga.run(7, mode=nv.BX_STEP_FUSED)
+    gb = _grid(prm, ic)
+    db = list(gb.run(2, mode=nv.BX_STEP_FUSED)) + list(gb.run(1, mode=nv.BX_STEP_FUSED)) + \
+        list(gb.run(4, mode=nv.BX_STEP_FUSED))
+    assert list(da) == db
+    oa, ob = ga.download(), gb.download()
+    for k in ("cons", "u", "p"):
+        assert _bits_equal(oa[k], ob[k]), k

@@ -10,7 +10,7 @@ import pyoracle as po
 from cases import Driver, OracleAdapter, case_inputs, ic_digest, sha
 
 CASES = ["sod_step1", "box12_outflow", "box12_walls", "box24_outflow", "stress8", "flatz_walls",
-         "riemann4_0", "riemann4_1", "riemann4_2", "riemann4_3", "sod_t02"]
+         "riemann4_0", "riemann4_1", "riemann4_2", "riemann4_3", "sod_t02", "readme_sod_y"]
 
 
 @pytest.mark.parametrize("name", CASES)
