@@ -61,6 +61,41 @@ def cpu_baseline(prm, ic, steps: int, threads: int):
                       f"step (oracle/boxzy_oracle.c)"}
 
 
+def mixed_study(prm, ic, steps, warmup, sync):
+    """The mixed form against the FP64 step on the same box: cell-updates/s of
+    each and the per-field L1/Linf error of the mixed state after `steps`."""
+    import numpy as np
+
+    from boxzyhydro_amd import ics
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+
+    res = {"workload": f"box {prm.nx - 4}^3, {steps} steps", "fields": {}}
+    states = {}
+    for name, prec in (("fp64", nv.BX_PREC_FP64), ("mixed", nv.BX_PREC_MIXED)):
+        g = HydroGrid(prm, ic, device=0, precision=prec)
+        g.run(1, mode=nv.BX_STEP_FUSED)  # first step from the set-up (exact)
+        g.sync()
+        t0 = time.perf_counter()
+        g.run(steps - 1, mode=nv.BX_STEP_FUSED)
+        g.sync()
+        dt = time.perf_counter() - t0
+        res[name] = {"cell_updates_per_s": prm.interior_cells * (steps - 1) / dt, "time": g.time,
+                     "fallbacks": g.fallback_count()}
+        states[name] = g.download()
+        del g
+    interior = ics.boundary_flags(prm).reshape(-1) == 0
+    a, b = states["fp64"], states["mixed"]
+    fields = {"rho": (a["cons"][:, 0], b["cons"][:, 0]), "p": (a["p"], b["p"]),
+              "mx": (a["cons"][:, 1], b["cons"][:, 1]), "E": (a["cons"][:, 4], b["cons"][:, 4])}
+    for k, (x, y) in fields.items():
+        x, y = x[interior], y[interior]
+        res["fields"][k] = {"l1_rel": float(np.abs(x - y).sum() / np.abs(x).sum()),
+                            "linf_rel": float(np.abs(x - y).max() / np.abs(x).max())}
+    res["speedup"] = res["mixed"]["cell_updates_per_s"] / res["fp64"]["cell_updates_per_s"]
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,10 +103,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--mode", choices=["fused", "exact"], default="fused")
-    ap.add_argument("--tile", type=int, nargs=2, default=None, metavar=("ROWS", "ZCHUNK"))
+    ap.add_argument("--tile", type=int, nargs=3, default=None, metavar=("YCHUNK", "ZCHUNK", "OCC"))
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision line")
+    ap.add_argument("--mixed-study", action="store_true",
+                    help="mixed form vs FP64 at --size for --steps steps: speed and per-field error")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -109,6 +147,9 @@ def main() -> int:
     prm = box_params(args.size)
     log(f"generating box {args.size}^3 ...")
     ic = ics.box_ic(prm, seed=0)
+    if args.mixed_study:
+        print(json.dumps(mixed_study(prm, ic, args.steps, args.warmup, sync)))
+        return 0
     mode = nv.BX_STEP_FUSED if args.mode == "fused" else nv.BX_STEP_EXACT
     g = HydroGrid(prm, ic, device=0)
     if args.tile:
@@ -165,6 +206,23 @@ def main() -> int:
     else:
         out["roofline"] = None
     del g
+    if not args.no_mixed:
+        log(f"mixed form: {args.steps} steps ...")
+        gm = HydroGrid(prm, ic, device=0, precision=nv.BX_PREC_MIXED)
+        gm.run(args.warmup, mode=nv.BX_STEP_FUSED)
+        gm.sync()
+        gm.profile(1)
+        t0 = time.perf_counter()
+        gm.run(args.steps, mode=nv.BX_STEP_FUSED)
+        gm.sync()
+        tm = time.perf_counter() - t0
+        m1, m2, mn = gm.profile(0)
+        vm = cells * args.steps / tm
+        out["mixed"] = {"value": vm, "unit": "cell-updates/s", "dtype": "f32 storage+reconstruction, f64 update",
+                        "speedup_vs_fp64": vm / value, "ms_per_step": tm * 1e3 / args.steps,
+                        "hbm_roofline_frac": vm * 100.0 / HBM_PEAK,
+                        "stage_avg_launch_ms": [m1 / max(mn, 1), m2 / max(mn, 1)]}
+        del gm
     if not args.no_cpu:
         log(f"cpu baseline ({args.cpu_steps} steps, {args.cpu_threads} threads) ...")
         out["cpu_baseline"] = cpu_baseline(prm, ic, args.cpu_steps, args.cpu_threads)

@@ -311,7 +311,7 @@ int bx_destroy(bx_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *dev[] = {c->cons, c->cons_old, c->u, c->p, c->adindx, c->qq, c->states, c->ghost_dst, c->ghost_src,
                    c->ghost_sign, c->ghost_scratch_src, c->ghost_scratch, c->ustar, c->pstar, c->unext, c->dsc,
-                   c->cons32, c->ustar32, c->unext32};
+                   c->cons32, c->ustar32, c->unext32, c->pstar32, c->eflux};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (c->hsc) (void)hipHostFree(c->hsc);
@@ -405,6 +405,12 @@ int bx_download(bx_ctx *c, double *cons, double *u, double *p, double *adindx)
     if (!c) return set_error(BX_ERR_ARG, "bx_download: null context");
     BX_HIP(hipSetDevice(c->device));
     int rc;
+    if (c->state_in_32) {
+        if ((rc = widen_cons(c))) return rc;
+        c->state_in_32 = false;
+        c->aux_stale = true;
+        c->fused_valid = false;
+    }
     if (c->aux_stale && (u || p || adindx)) {
         if ((rc = launch_materialize(c))) return rc;
         c->aux_stale = false;
@@ -430,6 +436,12 @@ int bx_download_cons_old(bx_ctx *c, double *cons_old)
 static int ensure_aux(bx_ctx *c)
 {
     BX_HIP(hipSetDevice(c->device));
+    if (c->state_in_32) {  // mixed form: bring the FP32 state back to the FP64 planes
+        int rc = widen_cons(c);
+        if (rc) return rc;
+        c->state_in_32 = false;
+        c->aux_stale = true;
+    }
     if (c->aux_stale) {
         int rc = launch_materialize(c);
         if (rc) return rc;
@@ -648,6 +660,10 @@ static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, d
                 continue;
             }
             c->fused_valid = true;
+            if (c->prm.precision == BX_PREC_MIXED && !c->state_in_32) {
+                if ((rc = narrow_cons(c))) return rc;
+                c->state_in_32 = true;
+            }
         }
         const double rate = rate_from_bits(c->hsc->rate_bits[0]);
         const double time0 = *time, dtold0 = *dtold;
@@ -662,6 +678,10 @@ static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, d
             *time = time0;
             *dtold = dtold0;
             *step = step0;
+            if (c->state_in_32) {
+                if ((rc = widen_cons(c))) return rc;
+                c->state_in_32 = false;
+            }
             if ((rc = launch_materialize(c))) return rc;
             c->aux_stale = false;
             c->fused_valid = false;
@@ -679,7 +699,8 @@ static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, d
             c->prof_ms[1] += t2;
             c->prof_steps++;
         }
-        std::swap(c->cons, c->unext);
+        if (c->state_in_32) std::swap(c->cons32, c->unext32);
+        else std::swap(c->cons, c->unext);
         c->aux_stale = true;
         c->old_stale = true;
         *time = *time + dt;  // main.f90:544-545 (dt*half*two == dt)
@@ -698,11 +719,13 @@ extern "C" int bx_step(bx_ctx *c, int32_t mode, int32_t nsteps, double endtime, 
     if (!c || !time || !dtold || !step) return set_error(BX_ERR_ARG, "bx_step: null argument");
     if (mode != BX_STEP_EXACT && mode != BX_STEP_FUSED) return set_error(BX_ERR_ARG, "bx_step: unknown mode");
     BX_HIP(hipSetDevice(c->device));
-    if (c->prm.precision != BX_PREC_FP64) return set_error(BX_ERR_UNSUPPORTED, "bx_step: precision not supported here");
+    if (c->prm.precision != BX_PREC_FP64 && mode != BX_STEP_FUSED)
+        return set_error(BX_ERR_UNSUPPORTED, "bx_step: the mixed form runs in fused mode only");
     int32_t k = 0;
     int rc = BX_OK;
     if (mode == BX_STEP_FUSED && !c->fused_ready && !c->fused_unsupported) {
         rc = fused_prepare(c);
+        if (rc == BX_ERR_UNSUPPORTED && c->prm.precision != BX_PREC_FP64) return rc;
         if (rc == BX_ERR_UNSUPPORTED) c->fused_unsupported = true;  // deep ghost chains: exact steps
         else if (rc) return rc;
         else c->fused_ready = true;
@@ -726,13 +749,14 @@ extern "C" int bx_step(bx_ctx *c, int32_t mode, int32_t nsteps, double endtime, 
     return BX_OK;
 }
 
-extern "C" int bx_tune(bx_ctx *c, int32_t row_tile, int32_t z_chunk)
+extern "C" int bx_tune(bx_ctx *c, int32_t y_chunk, int32_t z_chunk, int32_t occupancy)
 {
     if (!c) return set_error(BX_ERR_ARG, "bx_tune: null context");
-    if (row_tile != 4 && row_tile != 8 && row_tile != 12) return set_error(BX_ERR_ARG, "bx_tune: row tile 4, 8 or 12");
-    if (z_chunk < 1) return set_error(BX_ERR_ARG, "bx_tune: z chunk must be >= 1");
-    c->fused_ty = row_tile;
+    if (y_chunk < 1 || z_chunk < 1) return set_error(BX_ERR_ARG, "bx_tune: chunks must be >= 1");
+    if (occupancy != 0 && occupancy != 4) return set_error(BX_ERR_ARG, "bx_tune: occupancy 0 or 4");
+    c->fused_ly = y_chunk;
     c->fused_lz = z_chunk;
+    c->fused_occ = occupancy;
     return BX_OK;
 }
 

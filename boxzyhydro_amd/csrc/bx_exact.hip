@@ -41,7 +41,9 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long *dst)
     if (threadIdx.x == 0) {
         double m = wmax[0];
         for (int k = 1; k < (int)(blockDim.x >> 6); k++) m = vmax(m, wmax[k]);
-        atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+        // skip the atomic when the stored maximum is already larger
+        const unsigned long long b = (unsigned long long)__double_as_longlong(m);
+        if (b > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(dst, b);
     }
 }
 

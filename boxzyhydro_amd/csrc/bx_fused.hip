@@ -19,14 +19,14 @@
 // cell that would take an irregular branch raises a flag and the host redoes
 // the step with the exact sequence.
 //
-// Work decomposition (gfx950, wave64):
-//   * a wave holds 64 consecutive cells of one x-row; lanes 2..61 are output
-//     cells, lanes 0,1,62,63 the x-halo.  x-neighbours come through wave
-//     shuffles; each x-face is evaluated once (by the lane on its right).
-//   * a block stacks TY output rows plus 2 halo rows above and below (TY+4
-//     waves); y-neighbour data goes through LDS.
-//   * the block marches in z over a chunk of planes, keeping the primitive
-//     variables of planes k..k+2 and the z-face flux of plane k in registers.
+// Work decomposition (gfx950, wave64), two kernels per stage, every wave
+// independent (no LDS, no block barriers), each face evaluated once:
+//   * k_ydir: a wave holds 64 interior x-cells of one z-plane and marches in y
+//     with a register queue; it writes the y-term of fluxtmp (FP64 planes).
+//   * k_xz: a wave holds 64 cells of one interior row (lanes 2..61 output,
+//     0,1,62,63 the x-halo; x-neighbours through wave shuffles) and marches in
+//     z with a register queue; it adds the x- and z-terms in the reference's
+//     order and applies the stage update.
 #include <cmath>
 
 #include "bx_internal.h"
@@ -41,11 +41,11 @@ struct StageArgs {
     int nx, ny, nz;
     int flatz;
     int zlo, zhi, lz;     // interior z range [zlo,zhi) and planes per block
-    const double *uin;    // 5 planes: stage 1 U^n, stage 2 U*
-    const double *uold;   // stage 2: U^n
-    const double *pin;    // stage 2: p*
-    double *uout;         // 5 planes
-    double *pout;         // stage 1: p*
+    const void *uin;      // 5 planes: stage 1 U^n, stage 2 U*
+    const void *uold;     // stage 2: U^n
+    const void *pin;      // stage 2: p*
+    void *uout;           // 5 planes
+    void *pout;           // stage 1: p*
     double dt;
     double ds[3], inv_ds[3];
     int ds_pow2[3];
@@ -56,58 +56,64 @@ struct StageArgs {
     unsigned long long *rate;  // stage 2: next step's Courant rate (max)
 };
 
+template <typename T>
 struct W6 {
-    double d, u0, u1, u2, p, e;
+    T d, u0, u1, u2, p, e;
 };
 
-__device__ __forceinline__ double div_by(double x, double d, double inv, int pow2)
+template <typename T>
+__device__ __forceinline__ T div_by(T x, double d, double inv, int pow2)
 {
     // x/d; for d a power of two x*(1/d) is the same correctly rounded value
-    return pow2 ? x * inv : x / d;
+    return pow2 ? x * T(inv) : Arith<T>::div(x, T(d));
 }
 
-__device__ __forceinline__ double shfl_d(double v, int src)
+template <typename T>
+__device__ __forceinline__ T shfl_v(T v, int src)
 {
     return __shfl(v, src, 64);
 }
 
-__device__ __forceinline__ W6 shfl_w(const W6 &w, int src)
+template <typename T>
+__device__ __forceinline__ W6<T> shfl_w(const W6<T> &w, int src)
 {
-    W6 o;
-    o.d = shfl_d(w.d, src);
-    o.u0 = shfl_d(w.u0, src);
-    o.u1 = shfl_d(w.u1, src);
-    o.u2 = shfl_d(w.u2, src);
-    o.p = shfl_d(w.p, src);
-    o.e = shfl_d(w.e, src);
+    W6<T> o;
+    o.d = shfl_v(w.d, src);
+    o.u0 = shfl_v(w.u0, src);
+    o.u1 = shfl_v(w.u1, src);
+    o.u2 = shfl_v(w.u2, src);
+    o.p = shfl_v(w.p, src);
+    o.e = shfl_v(w.e, src);
     return o;
 }
 
-__device__ __forceinline__ double comp(const W6 &w, int j)
+template <typename T>
+__device__ __forceinline__ T comp(const W6<T> &w, int j)
 {
     return j == 0 ? w.u0 : (j == 1 ? w.u1 : w.u2);
 }
 
 // interface states of one cell in direction d (flux.f90:253-351, 391-479):
 // slopes from (wm, wc, wp) unless the cell is a ghost (zero slopes).
-template <int d>
-__device__ __forceinline__ void dir_states(const W6 &wm, const W6 &wc, const W6 &wp, double sound, bool ghost,
-                                           const StageArgs &a, Side<double> &mi, Side<double> &pl)
+template <int d, typename T>
+__device__ __forceinline__ void dir_states(const W6<T> &wm, const W6<T> &wc, const W6<T> &wp, T sound, bool ghost,
+                                           const StageArgs &a, Side<T> &mi, Side<T> &pl)
 {
-    double su0 = 0.0, su1 = 0.0, su2 = 0.0, sp = 0.0, sd = 0.0, se = 0.0;
+    T su0 = T(0), su1 = T(0), su2 = T(0), sp = T(0), sd = T(0), se = T(0);
     if (!ghost) {
-        su0 = slope(wm.u0, wc.u0, wp.u0);
-        su1 = slope(wm.u1, wc.u1, wp.u1);
-        su2 = slope(wm.u2, wc.u2, wp.u2);
-        sp = slope(wm.p, wc.p, wp.p);
-        sd = slope(wm.d, wc.d, wp.d);
-        se = slope(wm.e, wc.e, wp.e);
+        su0 = slope_finite(wm.u0, wc.u0, wp.u0);
+        su1 = slope_finite(wm.u1, wc.u1, wp.u1);
+        su2 = slope_finite(wm.u2, wc.u2, wp.u2);
+        sp = slope_finite(wm.p, wc.p, wp.p);
+        sd = slope_finite(wm.d, wc.d, wp.d);
+        se = slope_finite(wm.e, wc.e, wp.e);
     }
-    const double ud = comp(wc, d);
-    const double lp = div_by(vmax(ud + sound, 0.0) * a.dt, a.ds[d], a.inv_ds[d], a.ds_pow2[d]);
-    const double lm = div_by(vmin(ud - sound, 0.0) * a.dt, a.ds[d], a.inv_ds[d], a.ds_pow2[d]);
-    const double cm = 0.5 * (-1.0 - lm), cp = 0.5 * (1.0 - lp);
-    const double z = 0.0;  // transverse terms: 0*cc with cc=dt*half/ds > 0
+    const T ud = comp(wc, d);
+    const T dt = T(a.dt);
+    const T lp = div_by(vmax(ud + sound, T(0)) * dt, a.ds[d], a.inv_ds[d], a.ds_pow2[d]);
+    const T lm = div_by(vmin(ud - sound, T(0)) * dt, a.ds[d], a.inv_ds[d], a.ds_pow2[d]);
+    const T cm = T(0.5) * (T(-1) - lm), cp = T(0.5) * (T(1) - lp);
+    const T z = T(0);  // transverse terms: 0*cc with cc=dt*half/ds > 0
     mi.d = (wc.d + cm * sd) + z;
     pl.d = (wc.d + cp * sd) + z;
     mi.v[0] = (wc.u0 + cm * su0) + z;
@@ -120,254 +126,319 @@ __device__ __forceinline__ void dir_states(const W6 &wm, const W6 &wc, const W6 
     pl.p = (wc.p + cp * sp) + z;
     mi.e = (wc.e + cm * se) + z;
     pl.e = (wc.e + cp * se) + z;
-    mi.g = a.gam;  // slope(gamma,gamma,gamma)=0: gamma + (+-0) + 0 = gamma
-    pl.g = a.gam;
+    mi.g = T(a.gam);  // slope(gamma,gamma,gamma)=0: gamma + (+-0) + 0 = gamma
+    pl.g = T(a.gam);
 }
 
-__device__ __forceinline__ double sound_of(const W6 &w, double gam) { return sqrt(w.p * gam / w.d); }
+template <typename T>
+__device__ __forceinline__ T sound_of(const W6<T> &w, double gam)
+{
+    return Arith<T>::root(Arith<T>::div(w.p * T(gam), w.d));
+}
 
-// primitive variables of a loaded cell for each stage
-template <int STAGE>
-__device__ __forceinline__ W6 load_prims(const StageArgs &a, int64_t c, unsigned int &flg)
+// primitive variables of a loaded cell for each stage.
+// FP64 storage: the reference's own operations.  FP32 storage (mixed form):
+// the kinetic energy, the energy/pressure subtraction and u are formed in FP64
+// from the stored values, then rounded for the FP32 reconstruction.
+template <int STAGE, typename S>
+__device__ __forceinline__ W6<S> load_prims(const StageArgs &a, int64_t c, unsigned int &flg)
 {
     const int64_t n = a.n;
-    W6 w;
-    const double r = a.uin[c], m0 = a.uin[n + c], m1 = a.uin[2 * n + c], m2 = a.uin[3 * n + c];
-    const double E = a.uin[4 * n + c];
-    w.d = r;
-    w.u0 = m0 / r;
-    w.u1 = m1 / r;
-    w.u2 = m2 / r;
-    if (STAGE == 1) {
-        // source(0) energy re-round with g = -0 (source.f90:176-182), then
-        // state at main.f90:314 with u = m/rho, velocity at :318
-        const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r;
-        const double E1 = vmax(E - ek, a.seps) + ek;
-        w.e = vmax(E1 - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
-        w.p = w.e * a.gm1;
+    const S *uin = reinterpret_cast<const S *>(a.uin);
+    W6<S> w;
+    if constexpr (sizeof(S) == 8) {
+        const double r = uin[c], m0 = uin[n + c], m1 = uin[2 * n + c], m2 = uin[3 * n + c];
+        const double E = uin[4 * n + c];
+        w.d = r;
+        w.u0 = m0 / r;
+        w.u1 = m1 / r;
+        w.u2 = m2 / r;
+        if (STAGE == 1) {
+            // source(0) energy re-round with g = -0 (source.f90:176-182), then
+            // state at main.f90:314 with u = m/rho, velocity at :318
+            const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r;
+            const double E1 = vmax(E - ek, a.seps) + ek;
+            w.e = vmax(E1 - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
+            w.p = w.e * a.gm1;
+        } else {
+            // velocity at main.f90:355; p* is the state of :353 (stale velocity)
+            w.e = vmax(E - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
+            w.p = reinterpret_cast<const double *>(a.pin)[c];
+        }
+        if (fabs(w.u0) > a.vlimit || fabs(w.u1) > a.vlimit || fabs(w.u2) > a.vlimit) flg |= FLAG_CLAMP;
     } else {
-        // velocity at main.f90:355; p* is the state of :353 (stale velocity)
-        w.e = vmax(E - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
-        w.p = a.pin[c];
+        const double r = uin[c], m0 = uin[n + c], m1 = uin[2 * n + c], m2 = uin[3 * n + c];
+        const double E = uin[4 * n + c];
+        const double inv = 1.0 / r;
+        const double v0 = m0 * inv, v1 = m1 * inv, v2 = m2 * inv;
+        const double ek = 0.5 * (m0 * v0 + m1 * v1 + m2 * v2);
+        const double e = vmax(E - ek, a.seps);
+        w.d = (float)r;
+        w.u0 = (float)v0;
+        w.u1 = (float)v1;
+        w.u2 = (float)v2;
+        w.e = (float)e;
+        w.p = STAGE == 1 ? (float)(e * a.gm1) : reinterpret_cast<const float *>(a.pin)[c];
+        if (fabs(v0) > a.vlimit || fabs(v1) > a.vlimit || fabs(v2) > a.vlimit) flg |= FLAG_CLAMP;
     }
-    if (fabs(w.u0) > a.vlimit || fabs(w.u1) > a.vlimit || fabs(w.u2) > a.vlimit) flg |= FLAG_CLAMP;
     return w;
 }
 
-template <int STAGE, int TY>
-__global__ __launch_bounds__(64 * (TY + 4)) void k_stage(StageArgs a)
+// Per-wave reduction of a non-negative max into *dst; the atomic is skipped
+// when the stored maximum is already larger (almost always, after warm-up).
+__device__ __forceinline__ void wave_max_to(double v, unsigned long long *dst)
 {
-    constexpr int NR = TY + 4;
-    __shared__ double bufW[6][NR][64];       // W(k) of all rows
-    __shared__ double bufP[6][TY + 1][64];   // y-plus states of rows 1..TY+1
-    __shared__ double bufF[5][TY + 1][64];   // y-face fluxes below rows 2..TY+2
+    for (int off = 32; off > 0; off >>= 1) v = vmax(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0 && v > 0.0) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+        const unsigned long long cur = __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (b > cur) atomicMax(dst, b);
+    }
+}
 
+// ---------------------------------------------------------------------------
+// y-direction kernel: each wave holds 64 consecutive interior x-cells of one
+// z-plane and marches in y, keeping the primitives of rows j..j+2, the y-plus
+// state of row j and the face flux of j-1/2 in registers.  It writes the
+// y-term of fluxtmp, E = areaxz*(fy(2)-fy(1)) (flux.f90:189), in FP64.
+template <int STAGE, typename S>
+__global__ __launch_bounds__(256) void k_ydir(StageArgs a, double *__restrict__ Ey, int ly)
+{
     const int lane = threadIdx.x & 63;
-    const int r = threadIdx.x >> 6;  // wave row, 0..TY+3
+    const int w = threadIdx.x >> 6;
     const int nx = a.nx, ny = a.ny;
     const int64_t n = a.n, plane = (int64_t)nx * ny;
+    const int x = 2 + blockIdx.x * 64 + lane;
+    const bool xin = x <= nx - 3;
+    const int xc = xin ? x : nx - 3;
+    const int z = a.zlo + blockIdx.y * 4 + w;
+    if (z >= a.zhi) return;  // whole wave; no block-level synchronisation in this kernel
+    const int jb = 2 + blockIdx.z * ly;
+    const int je = min(jb + ly, ny - 2);
+    if (jb >= je) return;
+    const bool zg = !a.flatz && (z < 2 || z > a.nz - 3);
+    const int64_t base = (int64_t)z * plane + xc;
+    auto cell = [&](int j) -> int64_t { return base + (int64_t)j * nx; };
+    auto yghost = [&](int j) -> bool { return zg || j < 2 || j > ny - 3; };
+    unsigned int flg = 0;
 
-    const int x = blockIdx.x * kSegOut + lane;   // 0-based column
-    const int y = blockIdx.y * TY + r;           // 0-based row
-    const int xc = x < nx ? x : nx - 1;          // clamped for loads
-    const int yc = y < ny ? y : ny - 1;
-    const int64_t col = (int64_t)yc * nx + xc;
-    const bool row_out = (r >= 2) && (r < TY + 2) && (y <= ny - 3);
-    const bool is_out = row_out && lane >= 2 && lane < 62 && x >= 2 && x <= nx - 3;
-    const bool xy_ghost = (x < 2) || (x > nx - 3) || (y < 2) || (y > ny - 3);
+    W6<S> q0 = load_prims<STAGE, S>(a, cell(jb - 2), flg);
+    W6<S> q1 = load_prims<STAGE, S>(a, cell(jb - 1), flg);
+    W6<S> q2 = load_prims<STAGE, S>(a, cell(jb), flg);
+    W6<S> q3 = load_prims<STAGE, S>(a, cell(jb + 1), flg);
+    Side<S> mi, pl, yp;
+    dir_states<1>(q0, q1, q2, sound_of(q1, a.gam), yghost(jb - 1), a, mi, pl);
+    yp = pl;
+    dir_states<1>(q1, q2, q3, sound_of(q2, a.gam), yghost(jb), a, mi, pl);
+    S F0[5];
+    face_flux<S, 1>(yp, mi, F0);
+    yp = pl;
+    q0 = q2;
+    q1 = q3;
+    for (int j = jb; j < je; j++) {
+        q2 = load_prims<STAGE, S>(a, cell(j + 2), flg);
+        dir_states<1>(q0, q1, q2, sound_of(q1, a.gam), yghost(j + 1), a, mi, pl);
+        S F1[5];
+        face_flux<S, 1>(yp, mi, F1);
+        if (xin) {
+            const int64_t c = cell(j);
+            for (int i = 0; i < 5; i++) Ey[(int64_t)i * n + c] = a.axz * ((double)F1[i] - (double)F0[i]);
+        }
+        for (int i = 0; i < 5; i++) F0[i] = F1[i];
+        yp = pl;
+        q0 = q1;
+        q1 = q2;
+    }
+    if (flg) atomicOr(a.flags, flg);
+}
+
+// ---------------------------------------------------------------------------
+// x/z kernel: each wave holds 64 consecutive cells of one interior row
+// (lanes 2..61 are outputs, 0,1,62,63 the x-halo; x-neighbours through
+// shuffles) and marches in z with the primitives of planes k..k+2, the z-plus
+// state of plane k and the z-face flux of k-1/2 in registers.  It completes
+// fluxtmp with the y-term from k_ydir in the reference's summation order and
+// applies the stage update and everything pointwise up to the next stage.
+template <int STAGE, typename S, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const double *__restrict__ Ey)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int nx = a.nx;
+    const int64_t n = a.n, plane = (int64_t)nx * a.ny;
+    const S *uin = reinterpret_cast<const S *>(a.uin);
+    S *uout = reinterpret_cast<S *>(a.uout);
+
+    const int x = blockIdx.x * kSegOut + lane;  // 0-based column
+    const int y = 2 + blockIdx.y * 4 + w;       // interior row
+    if (y > a.ny - 3) return;                   // whole wave
+    const int xc = x < nx ? x : nx - 1;
+    const int64_t col = (int64_t)y * nx + xc;
+    const bool is_out = lane >= 2 && lane < 62 && x >= 2 && x <= nx - 3;
+    const bool x_ghost = (x < 2) || (x > nx - 3);
 
     const int kb = a.zlo + blockIdx.z * a.lz;
     const int ke = min(kb + a.lz, a.zhi);
-    if (kb >= ke) return;  // uniform across the block
+    if (kb >= ke) return;
 
     unsigned int flg = 0;
     double rmax = 0.0;
-    const int zs = a.flatz ? 0 : 1;  // z stride in planes
+    const S gamS = S(a.gam);
+    const int lm1 = (lane + 63) & 63, lp1 = (lane + 1) & 63;
 
     auto zplane = [&](int k) -> int64_t { return (int64_t)k * plane; };
     auto zghost = [&](int k) -> bool { return !a.flatz && (k < 2 || k > a.nz - 3); };
 
-    // z register queue: primitives of planes k, k+1, k+2 (flat z: all = k)
-    W6 q0, q1, q2;
-    q0 = load_prims<STAGE>(a, zplane(kb - 2 * zs) + col, flg);  // plane kb-2
-    q1 = load_prims<STAGE>(a, zplane(kb - zs) + col, flg);      // kb-1
-    q2 = load_prims<STAGE>(a, zplane(kb) + col, flg);           // kb
-    // z-plus state of plane kb-1 and face kb-1/2 need planes kb-2..kb+1
-    W6 q3 = load_prims<STAGE>(a, zplane(kb + zs) + col, flg);   // kb+1
-    Side<double> zp_prev, zmi, zpl;
-    double Fz_prev[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    W6<S> q0, q1, q2;
+    Side<S> zp_prev, zmi, zpl;
+    S Fz_prev[5] = {S(0), S(0), S(0), S(0), S(0)};
     if (!a.flatz) {
-        const bool g1 = xy_ghost || zghost(kb - zs);
-        dir_states<2>(q0, q1, q2, sound_of(q1, a.gam), g1, a, zmi, zpl);  // plane kb-1
+        q0 = load_prims<STAGE, S>(a, zplane(kb - 2) + col, flg);
+        q1 = load_prims<STAGE, S>(a, zplane(kb - 1) + col, flg);
+        q2 = load_prims<STAGE, S>(a, zplane(kb) + col, flg);
+        const W6<S> q3 = load_prims<STAGE, S>(a, zplane(kb + 1) + col, flg);
+        dir_states<2>(q0, q1, q2, sound_of(q1, a.gam), x_ghost || zghost(kb - 1), a, zmi, zpl);
         zp_prev = zpl;
-        const bool g2 = xy_ghost || zghost(kb);
-        dir_states<2>(q1, q2, q3, sound_of(q2, a.gam), g2, a, zmi, zpl);  // plane kb
-        face_flux<double, 2>(zp_prev, zmi, Fz_prev);
+        dir_states<2>(q1, q2, q3, sound_of(q2, a.gam), x_ghost || zghost(kb), a, zmi, zpl);
+        face_flux<S, 2>(zp_prev, zmi, Fz_prev);
         zp_prev = zpl;
+        q0 = q2;
+        q1 = q3;
     } else {
-        zp_prev.d = zp_prev.p = zp_prev.e = zp_prev.g = 0.0;
-        zp_prev.v[0] = zp_prev.v[1] = zp_prev.v[2] = 0.0;
+        q0 = load_prims<STAGE, S>(a, zplane(kb) + col, flg);
+        q1 = q0;
+        zp_prev.d = zp_prev.p = zp_prev.e = zp_prev.g = S(0);
+        zp_prev.v[0] = zp_prev.v[1] = zp_prev.v[2] = S(0);
     }
-    // queue now: planes kb, kb+1 in (q2, q3); shift so q0=kb, q1=kb+1
-    q0 = q2;
-    q1 = q3;
 
     for (int k = kb; k < ke; k++) {
-        // ---- leading plane k+2
-        q2 = load_prims<STAGE>(a, zplane(k + 2 * zs) + col, flg);
-        const bool ghost_k = xy_ghost || zghost(k);
-        const double snd = sound_of(q0, a.gam);
+        q2 = load_prims<STAGE, S>(a, zplane(a.flatz ? min(k + 1, ke - 1) : k + 2) + col, flg);
+        const bool ghost_k = x_ghost || zghost(k);
 
-        // ---- y direction through LDS
-        bufW[0][r][lane] = q0.d;
-        bufW[1][r][lane] = q0.u0;
-        bufW[2][r][lane] = q0.u1;
-        bufW[3][r][lane] = q0.u2;
-        bufW[4][r][lane] = q0.p;
-        bufW[5][r][lane] = q0.e;
-        __syncthreads();
-        Side<double> ymi, ypl;
-        if (r >= 1 && r <= TY + 2) {
-            W6 wm, wp;
-            wm.d = bufW[0][r - 1][lane];
-            wm.u0 = bufW[1][r - 1][lane];
-            wm.u1 = bufW[2][r - 1][lane];
-            wm.u2 = bufW[3][r - 1][lane];
-            wm.p = bufW[4][r - 1][lane];
-            wm.e = bufW[5][r - 1][lane];
-            wp.d = bufW[0][r + 1][lane];
-            wp.u0 = bufW[1][r + 1][lane];
-            wp.u1 = bufW[2][r + 1][lane];
-            wp.u2 = bufW[3][r + 1][lane];
-            wp.p = bufW[4][r + 1][lane];
-            wp.e = bufW[5][r + 1][lane];
-            dir_states<1>(wm, q0, wp, snd, ghost_k, a, ymi, ypl);
-            if (r <= TY + 1) {
-                bufP[0][r - 1][lane] = ypl.d;
-                bufP[1][r - 1][lane] = ypl.v[0];
-                bufP[2][r - 1][lane] = ypl.v[1];
-                bufP[3][r - 1][lane] = ypl.v[2];
-                bufP[4][r - 1][lane] = ypl.p;
-                bufP[5][r - 1][lane] = ypl.e;
-            }
+        // ---- z face k+1/2 from the register queue
+        S Fz1[5] = {S(0), S(0), S(0), S(0), S(0)};
+        Side<S> zpl1;
+        if (!a.flatz) {
+            Side<S> zmi1;
+            dir_states<2>(q0, q1, q2, sound_of(q1, a.gam), x_ghost || zghost(k + 1), a, zmi1, zpl1);
+            face_flux<S, 2>(zp_prev, zmi1, Fz1);
         }
-        __syncthreads();
-        if (r >= 2 && r <= TY + 2) {
-            Side<double> L;
-            L.d = bufP[0][r - 2][lane];
-            L.v[0] = bufP[1][r - 2][lane];
-            L.v[1] = bufP[2][r - 2][lane];
-            L.v[2] = bufP[3][r - 2][lane];
-            L.p = bufP[4][r - 2][lane];
-            L.e = bufP[5][r - 2][lane];
-            L.g = a.gam;
-            double Fy[5];
-            face_flux<double, 1>(L, ymi, Fy);
-            for (int i = 0; i < 5; i++) bufF[i][r - 2][lane] = Fy[i];
-        }
-        __syncthreads();
 
-        if (r >= 2 && r < TY + 2) {
-            // ---- x direction through shuffles
-            const int lm1 = (lane + 63) & 63, lp1 = (lane + 1) & 63;
-            const W6 wxm = shfl_w(q0, lm1);
-            const W6 wxp = shfl_w(q0, lp1);
-            Side<double> xmi, xpl;
+        // ---- x faces through shuffles
+        S dFx[5];
+        {
+            const S snd = sound_of(q0, a.gam);
+            const W6<S> wxm = shfl_w(q0, lm1);
+            const W6<S> wxp = shfl_w(q0, lp1);
+            Side<S> xmi, xpl;
             dir_states<0>(wxm, q0, wxp, snd, ghost_k, a, xmi, xpl);
-            Side<double> L;
-            L.d = shfl_d(xpl.d, lm1);
-            L.v[0] = shfl_d(xpl.v[0], lm1);
-            L.v[1] = shfl_d(xpl.v[1], lm1);
-            L.v[2] = shfl_d(xpl.v[2], lm1);
-            L.p = shfl_d(xpl.p, lm1);
-            L.e = shfl_d(xpl.e, lm1);
-            L.g = a.gam;
-            double Fx0[5];
-            face_flux<double, 0>(L, xmi, Fx0);  // face x-1/2
-            double Fx1[5];
-            for (int i = 0; i < 5; i++) Fx1[i] = shfl_d(Fx0[i], lp1);  // face x+1/2
+            Side<S> L;
+            L.d = shfl_v(xpl.d, lm1);
+            L.v[0] = shfl_v(xpl.v[0], lm1);
+            L.v[1] = shfl_v(xpl.v[1], lm1);
+            L.v[2] = shfl_v(xpl.v[2], lm1);
+            L.p = shfl_v(xpl.p, lm1);
+            L.e = shfl_v(xpl.e, lm1);
+            L.g = gamS;
+            S Fx0[5];
+            face_flux<S, 0>(L, xmi, Fx0);  // face x-1/2
+            for (int i = 0; i < 5; i++) dFx[i] = Fx0[i];
+        }
+        double ddx[5];
+        for (int i = 0; i < 5; i++) {
+            const S fx1 = shfl_v(dFx[i], lp1);  // face x+1/2
+            ddx[i] = (double)fx1 - (double)dFx[i];
+        }
 
-            // ---- z direction from the register queue: face k+1/2
-            Side<double> zmi1, zpl1;
-            double Fz1[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            if (!a.flatz) {
-                const bool ghost_k1 = xy_ghost || zghost(k + zs);
-                dir_states<2>(q0, q1, q2, sound_of(q1, a.gam), ghost_k1, a, zmi1, zpl1);
-                face_flux<double, 2>(zp_prev, zmi1, Fz1);
+        if (is_out) {
+            const int64_t c = zplane(k) + col;
+            double ft[5];
+            for (int i = 0; i < 5; i++) {
+                // flux.f90:188-190 in the reference's order, accumulated in FP64
+                const double dz = a.flatz ? 0.0 : ((double)Fz1[i] - (double)Fz_prev[i]);
+                const double s = (a.ayz * ddx[i] + Ey[(int64_t)i * n + c]) + a.axy * dz;
+                ft[i] = -div_by(s * a.dt, a.vol, a.inv_vol, a.vol_pow2);
             }
-
-            if (is_out) {
-                const int64_t c = zplane(k) + col;
-                double Fy1[5];
-                for (int i = 0; i < 5; i++) Fy1[i] = bufF[i][r - 1][lane];
-                double Fy0[5];
-                for (int i = 0; i < 5; i++) Fy0[i] = bufF[i][r - 2][lane];
-                double ft[5];
+            double U[5];
+            if (STAGE == 1) {
+                // cons = cons + fluxtmp (flux.f90:205) with cons = source(0)(U^n)
+                const double r0 = uin[c], m0 = uin[n + c], m1 = uin[2 * n + c], m2 = uin[3 * n + c];
+                const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r0;
+                const double E1 = vmax((double)uin[4 * n + c] - ek, a.seps) + ek;
+                U[0] = r0 + ft[0];
+                U[1] = m0 + ft[1];
+                U[2] = m1 + ft[2];
+                U[3] = m2 + ft[3];
+                U[4] = E1 + ft[4];
+            } else {
+                // cons = half*(cons_old+cons+fluxtmp) (flux.f90:213), cons_old = source(0)(U^n)
+                const S *uo = reinterpret_cast<const S *>(a.uold);
+                const double r0 = uo[c], m0 = uo[n + c], m1 = uo[2 * n + c], m2 = uo[3 * n + c];
+                const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r0;
+                const double E1 = vmax((double)uo[4 * n + c] - ek, a.seps) + ek;
+                const double old[5] = {r0, m0, m1, m2, E1};
+                for (int i = 0; i < 5; i++) U[i] = 0.5 * (old[i] + (double)uin[(int64_t)i * n + c] + ft[i]);
+            }
+            // floors, flux.f90:225-228
+            U[0] = vmax(U[0], a.srho);
+            if (U[4] < 0.0) U[4] = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0] + a.seps;
+            if (STAGE == 1) {
+                // p* = state() at main.f90:353 on U* with the velocity of :318
+                if (U[0] < 1e-4) flg |= FLAG_LOWRHO;
+                S Us[5];
+                for (int i = 0; i < 5; i++) Us[i] = (S)U[i];
+                const double r1 = Us[0], E1 = Us[4];
+                const double w0 = q0.u0, w1 = q0.u1, w2 = q0.u2;
+                const double ekin = r1 * 0.5 * (w0 * w0 + w1 * w1 + w2 * w2);
+                reinterpret_cast<S *>(a.pout)[c] = (S)(vmax(E1 - ekin, a.seps) * a.gm1);
+                for (int i = 0; i < 5; i++) uout[(int64_t)i * n + c] = Us[i];
+            } else {
+                // main.f90:374-515: state, velocity, source(1), state, velocity
+                if (U[0] < 1e-4) flg |= FLAG_LOWRHO;
+                const double v0 = U[1] / U[0], v1 = U[2] / U[0], v2 = U[3] / U[0];
+                if (fabs(v0) > a.vlimit || fabs(v1) > a.vlimit || fabs(v2) > a.vlimit) flg |= FLAG_CLAMP;
+                const double ek = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0];
+                U[4] = vmax(U[4] - ek, a.seps) + ek;
+                S Us[5];
                 for (int i = 0; i < 5; i++) {
-                    // flux.f90:188-190
-                    const double dz = a.flatz ? 0.0 : (Fz1[i] - Fz_prev[i]);
-                    const double s = (a.ayz * (Fx1[i] - Fx0[i]) + a.axz * (Fy1[i] - Fy0[i])) + a.axy * dz;
-                    ft[i] = -div_by(s * a.dt, a.vol, a.inv_vol, a.vol_pow2);
+                    Us[i] = (S)U[i];
+                    uout[(int64_t)i * n + c] = Us[i];
                 }
-                double U[5];
-                if (STAGE == 1) {
-                    // cons = cons + fluxtmp (flux.f90:205) with cons = source(0)(U^n)
-                    const double r0 = a.uin[c], m0 = a.uin[n + c], m1 = a.uin[2 * n + c], m2 = a.uin[3 * n + c];
-                    const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r0;
-                    const double E1 = vmax(a.uin[4 * n + c] - ek, a.seps) + ek;
-                    U[0] = r0 + ft[0];
-                    U[1] = m0 + ft[1];
-                    U[2] = m1 + ft[2];
-                    U[3] = m2 + ft[3];
-                    U[4] = E1 + ft[4];
-                } else {
-                    // cons = half*(cons_old+cons+fluxtmp) (flux.f90:213), cons_old = source(0)(U^n)
-                    const double r0 = a.uold[c], m0 = a.uold[n + c], m1 = a.uold[2 * n + c], m2 = a.uold[3 * n + c];
-                    const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r0;
-                    const double E1 = vmax(a.uold[4 * n + c] - ek, a.seps) + ek;
-                    const double old[5] = {r0, m0, m1, m2, E1};
-                    for (int i = 0; i < 5; i++) U[i] = 0.5 * (old[i] + a.uin[(int64_t)i * n + c] + ft[i]);
-                }
-                // floors, flux.f90:225-228
-                U[0] = vmax(U[0], a.srho);
-                if (U[4] < 0.0) U[4] = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0] + a.seps;
-                if (STAGE == 1) {
-                    // p* = state() at main.f90:353 on U* with the velocity of :318
-                    if (U[0] < 1e-4) flg |= FLAG_LOWRHO;
-                    const double ekin = U[0] * 0.5 * (q0.u0 * q0.u0 + q0.u1 * q0.u1 + q0.u2 * q0.u2);
-                    a.pout[c] = vmax(U[4] - ekin, a.seps) * a.gm1;
-                    for (int i = 0; i < 5; i++) a.uout[(int64_t)i * n + c] = U[i];
-                } else {
-                    // main.f90:374-515: state, velocity, source(1), state, velocity
-                    if (U[0] < 1e-4) flg |= FLAG_LOWRHO;
-                    const double v0 = U[1] / U[0], v1 = U[2] / U[0], v2 = U[3] / U[0];
-                    if (fabs(v0) > a.vlimit || fabs(v1) > a.vlimit || fabs(v2) > a.vlimit) flg |= FLAG_CLAMP;
-                    const double ek = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0];
-                    U[4] = vmax(U[4] - ek, a.seps) + ek;
-                    for (int i = 0; i < 5; i++) a.uout[(int64_t)i * n + c] = U[i];
-                    // next step: state (main.f90:232) and courant rate (courant.f90:20-21)
+                // next step: state (main.f90:232) and courant rate (courant.f90:20-21)
+                double p, w0, w1, w2, rr;
+                if constexpr (sizeof(S) == 8) {
+                    rr = U[0];
+                    w0 = v0;
+                    w1 = v1;
+                    w2 = v2;
                     const double ekin = U[0] * 0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
-                    const double p = vmax(U[4] - ekin, a.seps) * a.gm1;
-                    const double snd1 = sqrt(a.gam * p / U[0]);
-                    const double rt = (snd1 + fabs(v0)) / a.ds[0] + (snd1 + fabs(v1)) / a.ds[1] +
-                                      (snd1 + fabs(v2)) / a.ds[2];
-                    rmax = vmax(rmax, rt);
+                    p = vmax(U[4] - ekin, a.seps) * a.gm1;
+                } else {
+                    rr = Us[0];
+                    const double inv = 1.0 / rr;
+                    w0 = (double)Us[1] * inv;
+                    w1 = (double)Us[2] * inv;
+                    w2 = (double)Us[3] * inv;
+                    const double ekin = 0.5 * ((double)Us[1] * w0 + (double)Us[2] * w1 + (double)Us[3] * w2);
+                    p = vmax((double)Us[4] - ekin, a.seps) * a.gm1;
                 }
+                const double snd1 = sqrt(a.gam * p / rr);
+                const double rt =
+                    (snd1 + fabs(w0)) / a.ds[0] + (snd1 + fabs(w1)) / a.ds[1] + (snd1 + fabs(w2)) / a.ds[2];
+                rmax = vmax(rmax, rt);
             }
-            if (!a.flatz) {
-                for (int i = 0; i < 5; i++) Fz_prev[i] = Fz1[i];
-                zp_prev = zpl1;
-            }
+        }
+        if (!a.flatz) {
+            for (int i = 0; i < 5; i++) Fz_prev[i] = Fz1[i];
+            zp_prev = zpl1;
         }
         q0 = q1;
         q1 = q2;
+        if (a.flatz) q0 = q2;
     }
 
     if (flg) atomicOr(a.flags, flg);
-    if (STAGE == 2) {
-        for (int off = 32; off > 0; off >>= 1) rmax = vmax(rmax, __shfl_xor(rmax, off, 64));
-        if (lane == 0 && rmax > 0.0) atomicMax(a.rate, (unsigned long long)__double_as_longlong(rmax));
-    }
+    if (STAGE == 2) wave_max_to(rmax, a.rate);
 }
 
 // -------------------------------------------------------------- helpers
@@ -413,15 +484,15 @@ __global__ __launch_bounds__(256) void k_regular(int64_t n, const double *__rest
         rt = (snd + fabs(v0)) / dx + (snd + fabs(v1)) / dy + (snd + fabs(v2)) / dz;
     }
     if (flg) atomicOr(flags, flg);
-    for (int off = 32; off > 0; off >>= 1) rt = vmax(rt, __shfl_xor(rt, off, 64));
-    if ((threadIdx.x & 63) == 0 && rt > 0.0) atomicMax(rate, (unsigned long long)__double_as_longlong(rt));
+    wave_max_to(rt, rate);
 }
 
 // ghost fill of the 5 conserved planes and (optionally) p*, shallow tables only
+template <typename S>
 __global__ __launch_bounds__(256) void k_ghosts6(int64_t n, int64_t ng, const int32_t *__restrict__ dst,
                                                 const int32_t *__restrict__ src,
-                                                const int32_t *__restrict__ sign, double *__restrict__ planes,
-                                                double *__restrict__ pstar)
+                                                const int32_t *__restrict__ sign, S *__restrict__ planes,
+                                                S *__restrict__ pstar)
 {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ng) return;
@@ -429,11 +500,24 @@ __global__ __launch_bounds__(256) void k_ghosts6(int64_t n, int64_t ng, const in
     const int sg = sign[k];
     planes[d] = planes[s];
     for (int f = 1; f <= 3; f++) {
-        const double v = planes[f * n + s];
+        const S v = planes[f * n + s];
         planes[f * n + d] = ((sg >> (f - 1)) & 1) ? -v : v;
     }
     planes[4 * n + d] = planes[4 * n + s];
     if (pstar) pstar[d] = pstar[s];
+}
+
+// FP64 <-> FP32 copies of the 5 conserved planes (mixed form hand-over)
+__global__ __launch_bounds__(256) void k_narrow(int64_t m, const double *__restrict__ a, float *__restrict__ b)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) b[i] = (float)a[i];
+}
+
+__global__ __launch_bounds__(256) void k_widen(int64_t m, const float *__restrict__ a, double *__restrict__ b)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) b[i] = (double)a[i];
 }
 
 // ----------------------------------------------------------------- host
@@ -478,27 +562,29 @@ static StageArgs base_args(bx_ctx *c, double dt)
     return a;
 }
 
-template <int STAGE, int TY>
-static int launch_stage_ty(bx_ctx *c, const StageArgs &a)
-{
-    const unsigned gx = (unsigned)((c->nx - 4 + kSegOut - 1) / kSegOut);
-    const unsigned gy = (unsigned)((c->ny - 4 + TY - 1) / TY);
-    const int nzi = a.zhi - a.zlo;
-    const unsigned gz = (unsigned)((nzi + a.lz - 1) / a.lz);
-    hipLaunchKernelGGL((k_stage<STAGE, TY>), dim3(gx, gy, gz), dim3(64 * (TY + 4)), 0, c->stream, a);
-    BX_HIP(hipGetLastError());
-    return BX_OK;
-}
-
-template <int STAGE>
+template <int STAGE, typename S>
 static int launch_stage(bx_ctx *c, const StageArgs &a)
 {
-    switch (c->fused_ty) {
-    case 4: return launch_stage_ty<STAGE, 4>(c, a);
-    case 8: return launch_stage_ty<STAGE, 8>(c, a);
-    case 12: return launch_stage_ty<STAGE, 12>(c, a);
-    default: return set_error(BX_ERR_ARG, "fused: unsupported row tile");
+    const int nzi = a.zhi - a.zlo;
+    const int ly = c->fused_ly;
+    {
+        const unsigned gx = (unsigned)((c->nx - 4 + 63) / 64);
+        const unsigned gy = (unsigned)((nzi + 3) / 4);
+        const unsigned gz = (unsigned)((c->ny - 4 + ly - 1) / ly);
+        hipLaunchKernelGGL((k_ydir<STAGE, S>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, c->eflux, ly);
+        BX_HIP(hipGetLastError());
     }
+    {
+        const unsigned gx = (unsigned)((c->nx - 4 + kSegOut - 1) / kSegOut);
+        const unsigned gy = (unsigned)((c->ny - 4 + 3) / 4);
+        const unsigned gz = (unsigned)((nzi + a.lz - 1) / a.lz);
+        if (c->fused_occ >= 4)
+            hipLaunchKernelGGL((k_xz<STAGE, S, 4>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, c->eflux);
+        else
+            hipLaunchKernelGGL((k_xz<STAGE, S, 1>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, c->eflux);
+        BX_HIP(hipGetLastError());
+    }
+    return BX_OK;
 }
 
 int launch_materialize(bx_ctx *c)
@@ -516,11 +602,19 @@ int fused_prepare(bx_ctx *c)
         return set_error(BX_ERR_UNSUPPORTED, "fused step needs every ghost donor to be an interior cell");
     if (c->grav[0] != 0.0 || c->grav[1] != 0.0 || c->grav[2] != 0.0)
         return set_error(BX_ERR_UNSUPPORTED, "fused step assumes zero gravity");
+    if (c->prm.precision == BX_PREC_MIXED) {
+        const size_t pf = sizeof(float) * (size_t)c->n;
+        if (!c->cons32) BX_HIP(hipMalloc((void **)&c->cons32, 5 * pf));
+        if (!c->ustar32) BX_HIP(hipMalloc((void **)&c->ustar32, 5 * pf));
+        if (!c->unext32) BX_HIP(hipMalloc((void **)&c->unext32, 5 * pf));
+        if (!c->pstar32) BX_HIP(hipMalloc((void **)&c->pstar32, pf));
+    }
     const size_t pb = sizeof(double) * (size_t)c->n;
+    if (!c->eflux) BX_HIP(hipMalloc((void **)&c->eflux, 5 * pb));
     if (!c->ustar) BX_HIP(hipMalloc((void **)&c->ustar, 5 * pb));
     if (!c->unext) BX_HIP(hipMalloc((void **)&c->unext, 5 * pb));
     if (!c->pstar) BX_HIP(hipMalloc((void **)&c->pstar, pb));
-    if (c->fused_ty == 0) c->fused_ty = 8;
+    if (c->fused_ly == 0) c->fused_ly = 32;
     if (c->fused_lz == 0) c->fused_lz = 32;
     return BX_OK;
 }
@@ -538,42 +632,68 @@ int launch_regular_check(bx_ctx *c)
     return BX_OK;
 }
 
-static int ghosts6(bx_ctx *c, double *planes, double *pstar)
+template <typename S>
+static int ghosts6(bx_ctx *c, S *planes, S *pstar)
 {
     if (c->nghost == 0) return BX_OK;
-    hipLaunchKernelGGL(k_ghosts6, dim3(blocks_for(c->nghost, 256)), dim3(256), 0, c->stream, c->n, c->nghost,
+    hipLaunchKernelGGL(k_ghosts6<S>, dim3(blocks_for(c->nghost, 256)), dim3(256), 0, c->stream, c->n, c->nghost,
                        c->ghost_dst, c->ghost_src, c->ghost_sign, planes, pstar);
     BX_HIP(hipGetLastError());
     return BX_OK;
 }
 
-// One fused step from c->cons (U^n, regular) with full dt; result in c->unext.
-// The Courant rate of U^{n+1} accumulates into rate_bits[0] (zeroed here) and
-// irregular branches into flags (zeroed here).
-int fused_step(bx_ctx *c, double dt)
+int narrow_cons(bx_ctx *c)
+{
+    hipLaunchKernelGGL(k_narrow, dim3(blocks_for(5 * c->n, 256)), dim3(256), 0, c->stream, 5 * c->n, c->cons,
+                       c->cons32);
+    BX_HIP(hipGetLastError());
+    return BX_OK;
+}
+
+int widen_cons(bx_ctx *c)
+{
+    hipLaunchKernelGGL(k_widen, dim3(blocks_for(5 * c->n, 256)), dim3(256), 0, c->stream, 5 * c->n, c->cons32,
+                       c->cons);
+    BX_HIP(hipGetLastError());
+    return BX_OK;
+}
+
+// One fused step from U^n (c->cons, or c->cons32 in the mixed form; regular)
+// with full dt; result in c->unext / c->unext32.  The Courant rate of U^{n+1}
+// accumulates into rate_bits[0] and irregular branches into flags (both
+// zeroed here).
+template <typename S>
+static int fused_step_t(bx_ctx *c, double dt, S *u0, S *us, S *ps, S *un)
 {
     int rc;
-    BX_HIP(hipMemsetAsync(&c->dsc->flags, 0, sizeof(unsigned int) + 0, c->stream));
+    BX_HIP(hipMemsetAsync(&c->dsc->flags, 0, sizeof(unsigned int), c->stream));
     BX_HIP(hipMemsetAsync(&c->dsc->rate_bits[0], 0, sizeof(unsigned long long), c->stream));
     StageArgs a = base_args(c, dt);
-    a.uin = c->cons;
-    a.uout = c->ustar;
-    a.pout = c->pstar;
+    a.uin = u0;
+    a.uout = us;
+    a.pout = ps;
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[0], c->stream));
-    if ((rc = launch_stage<1>(c, a))) return rc;
+    if ((rc = launch_stage<1, S>(c, a))) return rc;
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[1], c->stream));
-    if ((rc = ghosts6(c, c->ustar, c->pstar))) return rc;
+    if ((rc = ghosts6<S>(c, us, ps))) return rc;
     StageArgs b = base_args(c, dt);
-    b.uin = c->ustar;
-    b.pin = c->pstar;
-    b.uold = c->cons;
-    b.uout = c->unext;
+    b.uin = us;
+    b.pin = ps;
+    b.uold = u0;
+    b.uout = un;
     b.rate = &c->dsc->rate_bits[0];
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[2], c->stream));
-    if ((rc = launch_stage<2>(c, b))) return rc;
+    if ((rc = launch_stage<2, S>(c, b))) return rc;
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[3], c->stream));
-    if ((rc = ghosts6(c, c->unext, nullptr))) return rc;
+    if ((rc = ghosts6<S>(c, un, (S *)nullptr))) return rc;
     return BX_OK;
+}
+
+int fused_step(bx_ctx *c, double dt)
+{
+    if (c->prm.precision == BX_PREC_MIXED)
+        return fused_step_t<float>(c, dt, c->cons32, c->ustar32, c->pstar32, c->unext32);
+    return fused_step_t<double>(c, dt, c->cons, c->ustar, c->pstar, c->unext);
 }
 
 }  // namespace bx

@@ -71,7 +71,9 @@ struct bx_ctx {
     bool fused_ready;  // fused buffers allocated, ghost table shallow
     bool fused_unsupported;
     bool ghosts_filled;  // ghost cells hold their donors' values (after any exact step)
-    int fused_ty, fused_lz;  // row tile and z chunk of the stage kernels
+    int fused_ly, fused_lz;  // y rows and z planes per wave of the stage kernels
+    int fused_occ;           // x/z kernel built for >= 4 waves/SIMD (register cap) when 4
+    double *eflux;           // 5 planes: y-term of fluxtmp (FP64)
     bool prof_on;
     hipEvent_t ev[4];
     double prof_ms[2];
@@ -84,7 +86,8 @@ struct bx_ctx {
     bx::DevScalars *hsc;   // pinned host mirror
 
     // mixed-precision planes (FP32 storage)
-    float *cons32, *ustar32, *unext32;
+    float *cons32, *ustar32, *unext32, *pstar32;
+    bool state_in_32;  // the current state lives in cons32 (mixed form)
 };
 
 namespace bx {
@@ -115,5 +118,7 @@ int launch_materialize(bx_ctx *c);
 int launch_regular_check(bx_ctx *c);
 int fused_step(bx_ctx *c, double dt);
 int fused_prepare(bx_ctx *c);
+int narrow_cons(bx_ctx *c);
+int widen_cons(bx_ctx *c);
 
 }  // namespace bx

@@ -11,6 +11,26 @@
 
 namespace bx {
 
+// Division and square root per arithmetic type: IEEE correctly rounded for
+// double (the reference's arithmetic); for float (the mixed-precision
+// reconstruction only) the hardware reciprocal and square root, ~1-2 ulp.
+template <typename T>
+struct Arith;
+template <>
+struct Arith<double> {
+    static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+    static __device__ __forceinline__ double root(double x) { return sqrt(x); }
+    static __device__ __forceinline__ double fmin(double a, double b) { return __builtin_fmin(a, b); }
+    static __device__ __forceinline__ double fmax(double a, double b) { return __builtin_fmax(a, b); }
+};
+template <>
+struct Arith<float> {
+    static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    static __device__ __forceinline__ float root(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float fmin(float a, float b) { return __builtin_fminf(a, b); }
+    static __device__ __forceinline__ float fmax(float a, float b) { return __builtin_fmaxf(a, b); }
+};
+
 template <typename T>
 __device__ __forceinline__ T vmax(T a, T b) { return a > b ? a : b; }
 template <typename T>
@@ -38,6 +58,25 @@ __device__ __forceinline__ T slope(T c0, T c1, T c2)
     return genminmod(tt * (c1 - c0), T(0.5) * (c2 - c0), tt * (c2 - c1));
 }
 
+// genminmod for finite (non-NaN) inputs with three min/max instructions:
+// "all three > 0" is exactly "their minimum > 0" and "all three < 0" is
+// "their maximum < 0"; min/max of same-signed nonzero values is exact.  For
+// finite data the result equals genminmod() above bit for bit.
+template <typename T>
+__device__ __forceinline__ T genminmod_finite(T x1, T x2, T x3)
+{
+    const T mn = Arith<T>::fmin(x1, Arith<T>::fmin(x2, x3));
+    const T mx = Arith<T>::fmax(x1, Arith<T>::fmax(x2, x3));
+    return mn > T(0) ? mn : (mx < T(0) ? mx : T(0));
+}
+
+template <typename T>
+__device__ __forceinline__ T slope_finite(T c0, T c1, T c2)
+{
+    const T tt = T(BX_THETA);
+    return genminmod_finite(tt * (c1 - c0), T(0.5) * (c2 - c0), tt * (c2 - c1));
+}
+
 // Primitive variables of one cell as the flux routine sees them.
 template <typename T>
 struct Prim {
@@ -60,7 +99,6 @@ __device__ __forceinline__ void muscl(T q, T lm, T lp, T s, T &minus, T &plus)
 
 // flux.f90:512-590 for one face: L = plus state of the cell below the face,
 // R = minus state of the cell above it, direction dir (0,1,2).
-// Accumulator type A for the flux itself (double for FP64 and mixed forms).
 template <typename T, int dir>
 __device__ __forceinline__ void face_flux(const Side<T> &L, const Side<T> &R, T F[5])
 {
@@ -90,14 +128,14 @@ __device__ __forceinline__ void face_flux(const Side<T> &L, const Side<T> &R, T 
     fr[3] = (dir == 2 ? R.p : zero) + sr[3] * vr;
     fr[4] = (sr[4] + R.p) * vr;
     // flux_hll, flux.f90:583-589
-    const T cr = sqrt(R.g * R.p / R.d);
-    const T cl = sqrt(L.g * L.p / L.d);
+    const T cr = Arith<T>::root(Arith<T>::div(R.g * R.p, R.d));
+    const T cl = Arith<T>::root(Arith<T>::div(L.g * L.p, L.d));
     const T ap = vmax(vmax(zero, vr + cr), vl + cl);
     const T am = vmax(vmax(zero, -(vr - cr)), -(vl - cl));
     const T apam = ap * am;
     const T den = ap + am;
 #pragma unroll
-    for (int i = 0; i < 5; i++) F[i] = (ap * fl[i] + am * fr[i] - apam * (sr[i] - sl[i])) / den;
+    for (int i = 0; i < 5; i++) F[i] = Arith<T>::div(ap * fl[i] + am * fr[i] - apam * (sr[i] - sl[i]), den);
 }
 
 }  // namespace bx

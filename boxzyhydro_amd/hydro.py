@@ -160,9 +160,10 @@ class HydroGrid:
         self.time, self.dtold, self.step_count = t.value, dto.value, st.value
         return dts[:taken.value].copy() if record_dt else np.zeros(0)
 
-    def tune(self, row_tile: int, z_chunk: int) -> None:
-        """Tile shape of the fused stage kernels (results are unchanged)."""
-        nv.check(nv.lib().bx_tune(self._h, row_tile, z_chunk), "bx_tune")
+    def tune(self, y_chunk: int, z_chunk: int, occupancy: int = 0) -> None:
+        """Rows per wave of the y-kernel, planes per wave of the x/z kernel and
+        its occupancy variant (0 or 4) for the fused step (results unchanged)."""
+        nv.check(nv.lib().bx_tune(self._h, y_chunk, z_chunk, occupancy), "bx_tune")
 
     def profile(self, enable: int = -1):
         """Event timing of the fused stage kernels: enable=1 start, 0 stop,

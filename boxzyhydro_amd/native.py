@@ -66,7 +66,7 @@ _SIGS = {
     "bx_step": [_VP, _I32, _I32, ctypes.c_double, _DP, _DP, ctypes.POINTER(_I32), _DP,
                 ctypes.POINTER(_I32)],
     "bx_fallback_count": [_VP, ctypes.POINTER(ctypes.c_int64)],
-    "bx_tune": [_VP, _I32, _I32],
+    "bx_tune": [_VP, _I32, _I32, _I32],
     "bx_profile": [_VP, _I32, _DP, ctypes.POINTER(ctypes.c_int64)],
     "bx_sync": [_VP],
 }
