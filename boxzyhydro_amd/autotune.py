@@ -1,8 +1,7 @@
 """Autotuner for the fused step's work split.
 
 Times a few steps of each candidate (y rows per wave of the y-direction
-kernel, z planes per wave of the x/z kernel, occupancy variant of the x/z
-kernel) on the grid it is given and keeps the fastest.  Results of the step do
+kernel, z planes per wave of the x/z kernel, variant of the x/z kernel) on the grid it is given and keeps the fastest.  Results of the step do
 not depend on the choice, so tuning can run on the production grid itself; the
 state it leaves behind is stepped forward by the timing steps, so tune on a
 copy (or re-upload the initial condition afterwards).
@@ -15,7 +14,7 @@ from typing import Iterable, List, Tuple
 
 from . import native as nv
 
-DEFAULT_CANDIDATES = list(itertools.product((16, 32, 64), (16, 32, 64), (0, 4)))
+DEFAULT_CANDIDATES = list(itertools.product((16, 32, 64), (16, 32, 64), (0, 1, 4)))
 
 
 def tune_step(grid, candidates: Iterable[Tuple[int, int, int]] = DEFAULT_CANDIDATES, steps: int = 3,

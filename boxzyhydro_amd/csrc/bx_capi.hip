@@ -753,7 +753,7 @@ extern "C" int bx_tune(bx_ctx *c, int32_t y_chunk, int32_t z_chunk, int32_t occu
 {
     if (!c) return set_error(BX_ERR_ARG, "bx_tune: null context");
     if (y_chunk < 1 || z_chunk < 1) return set_error(BX_ERR_ARG, "bx_tune: chunks must be >= 1");
-    if (occupancy != 0 && occupancy != 4) return set_error(BX_ERR_ARG, "bx_tune: occupancy 0 or 4");
+    if (occupancy != 0 && occupancy != 1 && occupancy != 4) return set_error(BX_ERR_ARG, "bx_tune: variant 0, 1 or 4");
     c->fused_ly = y_chunk;
     c->fused_lz = z_chunk;
     c->fused_occ = occupancy;

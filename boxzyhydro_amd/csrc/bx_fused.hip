@@ -36,6 +36,7 @@ namespace bx {
 
 constexpr int kSegOut = 60;  // output cells per wave row
 
+
 struct StageArgs {
     int64_t n;
     int nx, ny, nz;
@@ -136,19 +137,37 @@ __device__ __forceinline__ T sound_of(const W6<T> &w, double gam)
     return Arith<T>::root(Arith<T>::div(w.p * T(gam), w.d));
 }
 
+// Raw values of one cell as stored (conserved variables, and p* in stage 2).
+template <typename S>
+struct Raw {
+    S r, m0, m1, m2, E, p;
+};
+
+template <int STAGE, typename S>
+__device__ __forceinline__ Raw<S> load_raw(const StageArgs &a, int64_t c)
+{
+    const int64_t n = a.n;
+    const S *uin = reinterpret_cast<const S *>(a.uin);
+    Raw<S> v;
+    v.r = uin[c];
+    v.m0 = uin[n + c];
+    v.m1 = uin[2 * n + c];
+    v.m2 = uin[3 * n + c];
+    v.E = uin[4 * n + c];
+    v.p = STAGE == 2 ? reinterpret_cast<const S *>(a.pin)[c] : S(0);
+    return v;
+}
+
 // primitive variables of a loaded cell for each stage.
 // FP64 storage: the reference's own operations.  FP32 storage (mixed form):
 // the kinetic energy, the energy/pressure subtraction and u are formed in FP64
 // from the stored values, then rounded for the FP32 reconstruction.
 template <int STAGE, typename S>
-__device__ __forceinline__ W6<S> load_prims(const StageArgs &a, int64_t c, unsigned int &flg)
+__device__ __forceinline__ W6<S> prims(const StageArgs &a, const Raw<S> &v, unsigned int &flg)
 {
-    const int64_t n = a.n;
-    const S *uin = reinterpret_cast<const S *>(a.uin);
     W6<S> w;
     if constexpr (sizeof(S) == 8) {
-        const double r = uin[c], m0 = uin[n + c], m1 = uin[2 * n + c], m2 = uin[3 * n + c];
-        const double E = uin[4 * n + c];
+        const double r = v.r, m0 = v.m0, m1 = v.m1, m2 = v.m2, E = v.E;
         w.d = r;
         w.u0 = m0 / r;
         w.u1 = m1 / r;
@@ -163,12 +182,11 @@ __device__ __forceinline__ W6<S> load_prims(const StageArgs &a, int64_t c, unsig
         } else {
             // velocity at main.f90:355; p* is the state of :353 (stale velocity)
             w.e = vmax(E - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
-            w.p = reinterpret_cast<const double *>(a.pin)[c];
+            w.p = v.p;
         }
         if (fabs(w.u0) > a.vlimit || fabs(w.u1) > a.vlimit || fabs(w.u2) > a.vlimit) flg |= FLAG_CLAMP;
     } else {
-        const double r = uin[c], m0 = uin[n + c], m1 = uin[2 * n + c], m2 = uin[3 * n + c];
-        const double E = uin[4 * n + c];
+        const double r = v.r, m0 = v.m0, m1 = v.m1, m2 = v.m2, E = v.E;
         const double inv = 1.0 / r;
         const double v0 = m0 * inv, v1 = m1 * inv, v2 = m2 * inv;
         const double ek = 0.5 * (m0 * v0 + m1 * v1 + m2 * v2);
@@ -178,10 +196,16 @@ __device__ __forceinline__ W6<S> load_prims(const StageArgs &a, int64_t c, unsig
         w.u1 = (float)v1;
         w.u2 = (float)v2;
         w.e = (float)e;
-        w.p = STAGE == 1 ? (float)(e * a.gm1) : reinterpret_cast<const float *>(a.pin)[c];
+        w.p = STAGE == 1 ? (float)(e * a.gm1) : v.p;
         if (fabs(v0) > a.vlimit || fabs(v1) > a.vlimit || fabs(v2) > a.vlimit) flg |= FLAG_CLAMP;
     }
     return w;
+}
+
+template <int STAGE, typename S>
+__device__ __forceinline__ W6<S> load_prims(const StageArgs &a, int64_t c, unsigned int &flg)
+{
+    return prims<STAGE, S>(a, load_raw<STAGE, S>(a, c), flg);
 }
 
 // Per-wave reduction of a non-negative max into *dst; the atomic is skipped
@@ -200,9 +224,10 @@ __device__ __forceinline__ void wave_max_to(double v, unsigned long long *dst)
 // y-direction kernel: each wave holds 64 consecutive interior x-cells of one
 // z-plane and marches in y, keeping the primitives of rows j..j+2, the y-plus
 // state of row j and the face flux of j-1/2 in registers.  It writes the
-// y-term of fluxtmp, E = areaxz*(fy(2)-fy(1)) (flux.f90:189), in FP64.
+// y-term of fluxtmp, E = areaxz*(fy(2)-fy(1)) (flux.f90:189), in the storage
+// precision (FP64; FP32 in the mixed form, where the fluxes are FP32).
 template <int STAGE, typename S>
-__global__ __launch_bounds__(256) void k_ydir(StageArgs a, double *__restrict__ Ey, int ly)
+__global__ __launch_bounds__(256) void k_ydir(StageArgs a, S *__restrict__ Ey, int ly)
 {
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -235,14 +260,17 @@ __global__ __launch_bounds__(256) void k_ydir(StageArgs a, double *__restrict__ 
     yp = pl;
     q0 = q2;
     q1 = q3;
+    Raw<S> nxt = load_raw<STAGE, S>(a, cell(jb + 2));
     for (int j = jb; j < je; j++) {
-        q2 = load_prims<STAGE, S>(a, cell(j + 2), flg);
+        const Raw<S> cur = nxt;
+        nxt = load_raw<STAGE, S>(a, cell(j + 3));  // row j+3 <= ny-1: prefetch for the next row
+        q2 = prims<STAGE, S>(a, cur, flg);
         dir_states<1>(q0, q1, q2, sound_of(q1, a.gam), yghost(j + 1), a, mi, pl);
         S F1[5];
         face_flux<S, 1>(yp, mi, F1);
         if (xin) {
             const int64_t c = cell(j);
-            for (int i = 0; i < 5; i++) Ey[(int64_t)i * n + c] = a.axz * ((double)F1[i] - (double)F0[i]);
+            for (int i = 0; i < 5; i++) Ey[(int64_t)i * n + c] = (S)(a.axz * ((double)F1[i] - (double)F0[i]));
         }
         for (int i = 0; i < 5; i++) F0[i] = F1[i];
         yp = pl;
@@ -253,14 +281,15 @@ __global__ __launch_bounds__(256) void k_ydir(StageArgs a, double *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// x/z kernel: each wave holds 64 consecutive cells of one interior row
-// (lanes 2..61 are outputs, 0,1,62,63 the x-halo; x-neighbours through
-// shuffles) and marches in z with the primitives of planes k..k+2, the z-plus
+// x/z kernel: each wave holds 64 consecutive cells of the row-major x-y plane
+// (lanes 2..61 are outputs where interior, 0,1,62,63 the x-halo; a wave may
+// run across the end of one row into the next, whose ghost columns separate
+// them; x-neighbours through shuffles) and marches in z with the primitives of planes k..k+2, the z-plus
 // state of plane k and the z-face flux of k-1/2 in registers.  It completes
 // fluxtmp with the y-term from k_ydir in the reference's summation order and
 // applies the stage update and everything pointwise up to the next stage.
-template <int STAGE, typename S, int OCC>
-__global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const double *__restrict__ Ey)
+template <int STAGE, typename S, int OCC, bool PFE>
+__global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const S *__restrict__ Ey)
 {
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -269,13 +298,14 @@ __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const double *__re
     const S *uin = reinterpret_cast<const S *>(a.uin);
     S *uout = reinterpret_cast<S *>(a.uout);
 
-    const int x = blockIdx.x * kSegOut + lane;  // 0-based column
-    const int y = 2 + blockIdx.y * 4 + w;       // interior row
-    if (y > a.ny - 3) return;                   // whole wave
-    const int xc = x < nx ? x : nx - 1;
-    const int64_t col = (int64_t)y * nx + xc;
-    const bool is_out = lane >= 2 && lane < 62 && x >= 2 && x <= nx - 3;
-    const bool x_ghost = (x < 2) || (x > nx - 3);
+    const int64_t F0 = 2 * (int64_t)nx - 2 + (int64_t)(blockIdx.x * 4 + w) * kSegOut;  // lane 0
+    if (F0 + 2 >= (int64_t)(a.ny - 2) * nx) return;  // whole wave past the interior rows
+    int64_t F = F0 + lane;
+    if (F >= plane) F = plane - 1;  // halo lanes of the last wave: clamp for the loads
+    const int y = (int)(F / nx), x = (int)(F - (int64_t)y * nx);
+    const int64_t col = F;
+    const bool x_ghost = (x < 2) || (x > nx - 3) || (y < 2) || (y > a.ny - 3);
+    const bool is_out = lane >= 2 && lane < 62 && !x_ghost;
 
     const int kb = a.zlo + blockIdx.z * a.lz;
     const int ke = min(kb + a.lz, a.zhi);
@@ -290,6 +320,7 @@ __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const double *__re
     auto zghost = [&](int k) -> bool { return !a.flatz && (k < 2 || k > a.nz - 3); };
 
     W6<S> q0, q1, q2;
+    S cs0 = S(0);  // sound speed of plane k (q0), carried from the z-states of the previous plane
     Side<S> zp_prev, zmi, zpl;
     S Fz_prev[5] = {S(0), S(0), S(0), S(0), S(0)};
     if (!a.flatz) {
@@ -299,7 +330,8 @@ __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const double *__re
         const W6<S> q3 = load_prims<STAGE, S>(a, zplane(kb + 1) + col, flg);
         dir_states<2>(q0, q1, q2, sound_of(q1, a.gam), x_ghost || zghost(kb - 1), a, zmi, zpl);
         zp_prev = zpl;
-        dir_states<2>(q1, q2, q3, sound_of(q2, a.gam), x_ghost || zghost(kb), a, zmi, zpl);
+        cs0 = sound_of(q2, a.gam);
+        dir_states<2>(q1, q2, q3, cs0, x_ghost || zghost(kb), a, zmi, zpl);
         face_flux<S, 2>(zp_prev, zmi, Fz_prev);
         zp_prev = zpl;
         q0 = q2;
@@ -311,23 +343,36 @@ __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const double *__re
         zp_prev.v[0] = zp_prev.v[1] = zp_prev.v[2] = S(0);
     }
 
+    auto lead = [&](int k) -> int { return a.flatz ? min(k + 1, ke - 1) : min(k + 2, a.nz - 1); };
+    Raw<S> nxt = load_raw<STAGE, S>(a, zplane(lead(kb)) + col);
     for (int k = kb; k < ke; k++) {
-        q2 = load_prims<STAGE, S>(a, zplane(a.flatz ? min(k + 1, ke - 1) : k + 2) + col, flg);
+        const Raw<S> cur = nxt;
+        nxt = load_raw<STAGE, S>(a, zplane(lead(k + 1)) + col);  // prefetch the next leading plane
+        q2 = prims<STAGE, S>(a, cur, flg);
         const bool ghost_k = x_ghost || zghost(k);
+        // the y-term of this plane (written by k_ydir, not yet in cache): load early
+        const int64_t cu = zplane(k) + col;
+        double ey[5];
+        if (PFE)
+            for (int i = 0; i < 5; i++) ey[i] = Ey[(int64_t)i * n + cu];
 
         // ---- z face k+1/2 from the register queue
         S Fz1[5] = {S(0), S(0), S(0), S(0), S(0)};
         Side<S> zpl1;
+        S cs1 = S(0);
         if (!a.flatz) {
             Side<S> zmi1;
-            dir_states<2>(q0, q1, q2, sound_of(q1, a.gam), x_ghost || zghost(k + 1), a, zmi1, zpl1);
+            cs1 = sound_of(q1, a.gam);
+            dir_states<2>(q0, q1, q2, cs1, x_ghost || zghost(k + 1), a, zmi1, zpl1);
             face_flux<S, 2>(zp_prev, zmi1, Fz1);
+        } else {
+            cs0 = sound_of(q0, a.gam);
         }
 
         // ---- x faces through shuffles
         S dFx[5];
         {
-            const S snd = sound_of(q0, a.gam);
+            const S snd = cs0;
             const W6<S> wxm = shfl_w(q0, lm1);
             const W6<S> wxp = shfl_w(q0, lp1);
             Side<S> xmi, xpl;
@@ -353,18 +398,25 @@ __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const double *__re
         if (is_out) {
             const int64_t c = zplane(k) + col;
             double ft[5];
+            if (!PFE)
+                for (int i = 0; i < 5; i++) ey[i] = Ey[(int64_t)i * n + c];
+            double ui[5], uo[5];
+            for (int i = 0; i < 5; i++) {
+                ui[i] = uin[(int64_t)i * n + c];
+                uo[i] = STAGE == 2 ? (double)reinterpret_cast<const S *>(a.uold)[(int64_t)i * n + c] : 0.0;
+            }
             for (int i = 0; i < 5; i++) {
                 // flux.f90:188-190 in the reference's order, accumulated in FP64
                 const double dz = a.flatz ? 0.0 : ((double)Fz1[i] - (double)Fz_prev[i]);
-                const double s = (a.ayz * ddx[i] + Ey[(int64_t)i * n + c]) + a.axy * dz;
+                const double s = (a.ayz * ddx[i] + ey[i]) + a.axy * dz;
                 ft[i] = -div_by(s * a.dt, a.vol, a.inv_vol, a.vol_pow2);
             }
             double U[5];
             if (STAGE == 1) {
                 // cons = cons + fluxtmp (flux.f90:205) with cons = source(0)(U^n)
-                const double r0 = uin[c], m0 = uin[n + c], m1 = uin[2 * n + c], m2 = uin[3 * n + c];
+                const double r0 = ui[0], m0 = ui[1], m1 = ui[2], m2 = ui[3];
                 const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r0;
-                const double E1 = vmax((double)uin[4 * n + c] - ek, a.seps) + ek;
+                const double E1 = vmax(ui[4] - ek, a.seps) + ek;
                 U[0] = r0 + ft[0];
                 U[1] = m0 + ft[1];
                 U[2] = m1 + ft[2];
@@ -372,12 +424,11 @@ __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const double *__re
                 U[4] = E1 + ft[4];
             } else {
                 // cons = half*(cons_old+cons+fluxtmp) (flux.f90:213), cons_old = source(0)(U^n)
-                const S *uo = reinterpret_cast<const S *>(a.uold);
-                const double r0 = uo[c], m0 = uo[n + c], m1 = uo[2 * n + c], m2 = uo[3 * n + c];
+                const double r0 = uo[0], m0 = uo[1], m1 = uo[2], m2 = uo[3];
                 const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r0;
-                const double E1 = vmax((double)uo[4 * n + c] - ek, a.seps) + ek;
+                const double E1 = vmax(uo[4] - ek, a.seps) + ek;
                 const double old[5] = {r0, m0, m1, m2, E1};
-                for (int i = 0; i < 5; i++) U[i] = 0.5 * (old[i] + (double)uin[(int64_t)i * n + c] + ft[i]);
+                for (int i = 0; i < 5; i++) U[i] = 0.5 * (old[i] + ui[i] + ft[i]);
             }
             // floors, flux.f90:225-228
             U[0] = vmax(U[0], a.srho);
@@ -431,6 +482,7 @@ __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const double *__re
         if (!a.flatz) {
             for (int i = 0; i < 5; i++) Fz_prev[i] = Fz1[i];
             zp_prev = zpl1;
+            cs0 = cs1;
         }
         q0 = q1;
         q1 = q2;
@@ -571,17 +623,21 @@ static int launch_stage(bx_ctx *c, const StageArgs &a)
         const unsigned gx = (unsigned)((c->nx - 4 + 63) / 64);
         const unsigned gy = (unsigned)((nzi + 3) / 4);
         const unsigned gz = (unsigned)((c->ny - 4 + ly - 1) / ly);
-        hipLaunchKernelGGL((k_ydir<STAGE, S>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, c->eflux, ly);
+        hipLaunchKernelGGL((k_ydir<STAGE, S>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, (S *)c->eflux, ly);
         BX_HIP(hipGetLastError());
     }
     {
-        const unsigned gx = (unsigned)((c->nx - 4 + kSegOut - 1) / kSegOut);
-        const unsigned gy = (unsigned)((c->ny - 4 + 3) / 4);
+        const int64_t nwaves = ((int64_t)(c->ny - 4) * c->nx + kSegOut - 1) / kSegOut;
+        const unsigned gx = (unsigned)((nwaves + 3) / 4);
+        const unsigned gy = 1;
         const unsigned gz = (unsigned)((nzi + a.lz - 1) / a.lz);
-        if (c->fused_occ >= 4)
-            hipLaunchKernelGGL((k_xz<STAGE, S, 4>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, c->eflux);
+        const S *ey = (const S *)c->eflux;
+        if (c->fused_occ == 4)
+            hipLaunchKernelGGL((k_xz<STAGE, S, 4, false>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, ey);
+        else if (c->fused_occ == 1)
+            hipLaunchKernelGGL((k_xz<STAGE, S, 1, true>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, ey);
         else
-            hipLaunchKernelGGL((k_xz<STAGE, S, 1>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, c->eflux);
+            hipLaunchKernelGGL((k_xz<STAGE, S, 3, false>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, ey);
         BX_HIP(hipGetLastError());
     }
     return BX_OK;
@@ -614,8 +670,13 @@ int fused_prepare(bx_ctx *c)
     if (!c->ustar) BX_HIP(hipMalloc((void **)&c->ustar, 5 * pb));
     if (!c->unext) BX_HIP(hipMalloc((void **)&c->unext, 5 * pb));
     if (!c->pstar) BX_HIP(hipMalloc((void **)&c->pstar, pb));
-    if (c->fused_ly == 0) c->fused_ly = 32;
-    if (c->fused_lz == 0) c->fused_lz = 32;
+    // defaults from the autotuner on the 256^3 box (FP64: 64 rows, 32 planes;
+    // mixed: 64 rows, 16 planes, y-term loaded early)
+    if (c->fused_ly == 0) c->fused_ly = 64;
+    if (c->fused_lz == 0) {
+        c->fused_lz = c->prm.precision == BX_PREC_MIXED ? 16 : 32;
+        c->fused_occ = c->prm.precision == BX_PREC_MIXED ? 1 : 0;
+    }
     return BX_OK;
 }
 

@@ -104,10 +104,10 @@ int bx_step(bx_ctx *ctx, int32_t mode, int32_t nsteps, double endtime, double *t
 int bx_fallback_count(const bx_ctx *ctx, int64_t *count);
 
 /* Work split of the fused stage kernels: y rows per wave of the y-direction
- * kernel, z planes per wave of the x/z kernel, and occupancy 4 to use the
- * x/z kernel built for four waves per SIMD (0: the compiler's choice).  Used
- * by the autotuner; results do not change. */
-int bx_tune(bx_ctx *ctx, int32_t y_chunk, int32_t z_chunk, int32_t occupancy);
+ * kernel, z planes per wave of the x/z kernel, and the x/z kernel variant
+ * (0: default; 1: y-term loaded one plane early; 4: built for four waves per
+ * SIMD).  Used by the autotuner; results do not change. */
+int bx_tune(bx_ctx *ctx, int32_t y_chunk, int32_t z_chunk, int32_t variant);
 
 /* Event timing of the fused stage kernels on the context's stream.
  * enable=1 starts (and clears), 0 stops, -1 only reads.  stage_ms[2]
