@@ -97,9 +97,21 @@ __device__ __forceinline__ void muscl(T q, T lm, T lp, T s, T &minus, T &plus)
     plus = q + T(0.5) * (T(1) - lp) * s;
 }
 
+// a/b for double, where a zero numerator is answered without the division:
+// for finite nonzero b, IEEE gives the signed zero of sign(a) xor sign(b),
+// which is exactly a*b.  Used where most numerators are exactly +-0 (the
+// transverse components of a one-dimensional flow), so whole waves skip the
+// correctly rounded division sequence.
+__device__ __forceinline__ double qdiv(double a, double b)
+{
+    if (a == 0.0 && b != 0.0 && __builtin_isfinite(b)) return a * b;
+    return a / b;
+}
+
 // flux.f90:512-590 for one face: L = plus state of the cell below the face,
-// R = minus state of the cell above it, direction dir (0,1,2).
-template <typename T, int dir>
+// R = minus state of the cell above it, direction dir (0,1,2).  ZSKIP: the
+// final quotients go through qdiv (same values).
+template <typename T, int dir, bool ZSKIP = false>
 __device__ __forceinline__ void face_flux(const Side<T> &L, const Side<T> &R, T F[5])
 {
     const T h = T(0.5);
@@ -135,7 +147,13 @@ __device__ __forceinline__ void face_flux(const Side<T> &L, const Side<T> &R, T 
     const T apam = ap * am;
     const T den = ap + am;
 #pragma unroll
-    for (int i = 0; i < 5; i++) F[i] = Arith<T>::div(ap * fl[i] + am * fr[i] - apam * (sr[i] - sl[i]), den);
+    for (int i = 0; i < 5; i++) {
+        const T num = ap * fl[i] + am * fr[i] - apam * (sr[i] - sl[i]);
+        if constexpr (ZSKIP && sizeof(T) == 8)
+            F[i] = qdiv(num, den);
+        else
+            F[i] = Arith<T>::div(num, den);
+    }
 }
 
 }  // namespace bx

@@ -122,10 +122,27 @@ def box_ic(prm: HydroParams, seed: int = 0, sigma_cells: float = 4.0,
     return _assemble(prm, rho, vel[0], vel[1], vel[2], p)
 
 
+def _riemann_draws(n_problems: int, seed: int, lmin: int, lmax: int):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lens = np.rint(np.exp(rng.uniform(np.log(lmin), np.log(lmax), n_problems))).astype(int)
+    lr = np.exp(rng.uniform(np.log(0.1), np.log(10.0), (n_problems, 4)))
+    vv = rng.uniform(-1.0, 1.0, (n_problems, 2))
+    return lens, lr, vv
+
+
+def _riemann_params(L: int, endtime: float, gamma: float, vlimit: float) -> HydroParams:
+    d = 1.0 / L
+    return HydroParams(nx=L + 4, ny=8, nz=4, dx=d, dy=d, dz=d,
+                       no_outflow_xl=False, no_outflow_xr=False, no_outflow_yl=False,
+                       no_outflow_yr=False, no_outflow_zl=False, no_outflow_zr=False,
+                       small_rho=1e-12, small_eps=1e-40, vlimit=vlimit, endtime=endtime,
+                       dtout=1e30, cfl=0.5, gammafix=gamma)
+
+
 def riemann_batch(n_problems: int = 512, seed: int = 0, lmin: int = 64, lmax: int = 4096,
                   endtime: float = 0.1, gamma: float = 1.4,
                   vlimit: float = 1e3) -> List[Tuple[HydroParams, Dict[str, np.ndarray]]]:
-    """Independent 1-D Riemann problems of uneven length.
+    """Independent 1-D Riemann problems of uneven length, as full 3-D grids.
 
     Lengths are log-uniform in [lmin, lmax] cells (each L x 4 x 4 interior,
     dx = 1/L so every tube spans [0,1]); left/right rho and p log-uniform in
@@ -133,19 +150,10 @@ def riemann_batch(n_problems: int = 512, seed: int = 0, lmin: int = 64, lmax: in
     (x=0.5 of [0,1]).  ``vlimit`` is raised above the reference default so the
     speed clamp (velocity.f90:33-50) does not act on strong rarefactions.
     """
-    rng = np.random.Generator(np.random.PCG64(seed))
-    lens = np.rint(np.exp(rng.uniform(np.log(lmin), np.log(lmax), n_problems))).astype(int)
-    lr = np.exp(rng.uniform(np.log(0.1), np.log(10.0), (n_problems, 4)))
-    vv = rng.uniform(-1.0, 1.0, (n_problems, 2))
+    lens, lr, vv = _riemann_draws(n_problems, seed, lmin, lmax)
     out = []
     for i in range(n_problems):
-        L = int(lens[i])
-        d = 1.0 / L
-        prm = HydroParams(nx=L + 4, ny=8, nz=4, dx=d, dy=d, dz=d,
-                          no_outflow_xl=False, no_outflow_xr=False, no_outflow_yl=False,
-                          no_outflow_yr=False, no_outflow_zl=False, no_outflow_zr=False,
-                          small_rho=1e-12, small_eps=1e-40, vlimit=vlimit, endtime=endtime,
-                          dtout=1e30, cfl=0.5, gammafix=gamma)
+        prm = _riemann_params(int(lens[i]), endtime, gamma, vlimit)
         x, _, _ = cell_centres(prm)
         left = np.broadcast_to((x < 0.0)[None, None, :], prm.shape)
         rho = np.where(left, lr[i, 0], lr[i, 1])
@@ -153,6 +161,44 @@ def riemann_batch(n_problems: int = 512, seed: int = 0, lmin: int = 64, lmax: in
         vx = np.where(left, vv[i, 0], vv[i, 1])
         zero = np.zeros(prm.shape)
         out.append((prm, _assemble(prm, rho, vx, zero, zero, p)))
+    return out
+
+
+def tube_row(prm: HydroParams, rho, vx, p) -> Dict[str, np.ndarray]:
+    """The x-row of a shock-tube set-up (every (y,z) row of the grid equal),
+    by the reference convention: E = p/(gamma-1) + kinetic, and the frame
+    cells (ix = 1, nx) rho=small_rho, m=0, E=small_eps with the row's u, p."""
+    nx = prm.nx
+    rho = np.asarray(rho, np.float64).reshape(nx)
+    vx = np.asarray(vx, np.float64).reshape(nx)
+    p = np.asarray(p, np.float64).reshape(nx).copy()
+    gam = prm.gammafix
+    zero = np.zeros(nx)
+    cons = np.empty((nx, 5), np.float64)
+    cons[:, 0] = rho
+    cons[:, 1] = rho * vx
+    cons[:, 2] = rho * zero
+    cons[:, 3] = rho * zero
+    cons[:, 4] = p / (gam - 1.0) + rho * 0.5 * ((vx * vx + zero * zero) + zero * zero)
+    for c in (0, nx - 1):
+        cons[c] = (prm.small_rho, 0.0, 0.0, 0.0, prm.small_eps)
+    u = np.ascontiguousarray(np.stack([vx, zero, zero], axis=1))
+    return {"cons": cons, "u": u, "p": p}
+
+
+def riemann_tubes(n_problems: int = 512, seed: int = 0, lmin: int = 64, lmax: int = 4096,
+                  endtime: float = 0.1, gamma: float = 1.4,
+                  vlimit: float = 1e3) -> List[Tuple[HydroParams, Dict[str, np.ndarray]]]:
+    """The problems of :func:`riemann_batch` as tube rows (bit-identical to
+    the rows of the 3-D set-ups), without building the 3-D grids."""
+    lens, lr, vv = _riemann_draws(n_problems, seed, lmin, lmax)
+    out = []
+    for i in range(n_problems):
+        prm = _riemann_params(int(lens[i]), endtime, gamma, vlimit)
+        x, _, _ = cell_centres(prm)
+        left = x < 0.0
+        out.append((prm, tube_row(prm, np.where(left, lr[i, 0], lr[i, 1]),
+                                  np.where(left, vv[i, 0], vv[i, 1]), np.where(left, lr[i, 2], lr[i, 3]))))
     return out
 
 

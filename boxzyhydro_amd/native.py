@@ -69,6 +69,17 @@ _SIGS = {
     "bx_tune": [_VP, _I32, _I32, _I32],
     "bx_profile": [_VP, _I32, _DP, ctypes.POINTER(ctypes.c_int64)],
     "bx_sync": [_VP],
+    "bx_batch_check": [_I32, ctypes.POINTER(BxParams), _DP, _DP, ctypes.POINTER(_I32)],
+    "bx_batch_create": [_I32, ctypes.POINTER(BxParams), _DP, _DP, _DP, _DP, _I32, ctypes.POINTER(_VP)],
+    "bx_batch_destroy": [_VP],
+    "bx_batch_cells": [_VP, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
+    "bx_batch_order": [_VP, ctypes.POINTER(_I32)],
+    "bx_batch_tune": [_VP, _I32, _I32],
+    "bx_batch_log_dt": [_VP, _I32],
+    "bx_batch_run": [_VP, _I32, _I32],
+    "bx_batch_sync": [_VP, _DP],
+    "bx_batch_result": [_VP, _DP, _DP, _DP, _DP, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_uint32)],
+    "bx_batch_dts": [_VP, _DP],
 }
 
 EXPORTED = tuple(_SIGS) + ("bx_last_error",)

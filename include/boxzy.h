@@ -114,6 +114,66 @@ int bx_tune(bx_ctx *ctx, int32_t y_chunk, int32_t z_chunk, int32_t variant);
  * receives the summed durations of stage 1 and stage 2 over *steps steps. */
 int bx_profile(bx_ctx *ctx, int32_t enable, double *stage_ms, int64_t *steps);
 
+/* ------------------------------------------------------------------------
+ * Batched shock tubes.  The reference runs one grid per process; this entry
+ * family runs many independent tubes in one persistent launch (the queue of
+ * tubes is ordered by estimated cost, largest first), optionally replayed
+ * from a captured hipGraph.  There is no reference counterpart to cite: each
+ * tube is what main.f90:211-555 computes for a grid that
+ *   - has nz < 6 (flat z, init_grid.f90:241-247), outflow on every side;
+ *   - holds the same set-up in every (y,z) row, except the frame cells
+ *     (boundary=1), which hold rho=small_rho, m=0, E=small_eps with the row's
+ *     u and p (init_conditions.f90:60-68).
+ * A tube is passed as that row: nx cells of cons(5,nx), u(3,nx), p(nx)
+ * (Fortran layout), rows of all tubes concatenated in order.  The result is
+ * bit-identical to the reference run of the full grid (every row of which
+ * equals the returned row, ghost cells holding their donors' values).
+ */
+typedef struct bx_batch bx_batch;
+
+/* Host-only check that every tube meets the row form: flat z, outflow,
+ * FP64, nx >= 8, set-up density >= 1e-4 outside the frame, ghost set-up
+ * velocities equal to their donors'.  *bad receives the first failing tube
+ * (or -1). */
+int bx_batch_check(int32_t ntube, const bx_params *prm, const double *cons_rows, const double *u_rows,
+                   int32_t *bad);
+
+/* Upload the tubes; tube i runs until time >= endtime[i] (main.f90:212). */
+int bx_batch_create(int32_t ntube, const bx_params *prm, const double *endtime, const double *cons_rows,
+                    const double *u_rows, const double *p_rows, int32_t device, bx_batch **out);
+int bx_batch_destroy(bx_batch *b);
+
+/* Cells of all rows, and interior cells of the full grids the rows stand for. */
+int bx_batch_cells(const bx_batch *b, int64_t *row_cells, int64_t *interior_cells);
+
+/* Queue order (tube indices, largest estimated cost first). */
+int bx_batch_order(const bx_batch *b, int32_t *order);
+
+/* Work split: waves per workgroup (4, 8, 16) and workgroups per compute unit
+ * (1..8) of the persistent launch.  Used by the autotuner; results do not
+ * change. */
+int bx_batch_tune(bx_batch *b, int32_t waves_per_block, int32_t blocks_per_cu);
+
+/* Keep the first maxlog dt values of every tube (0: off). */
+int bx_batch_log_dt(bx_batch *b, int32_t maxlog);
+
+/* Run every tube from its set-up to its endtime (at most maxsteps steps,
+ * 1000000 in the reference) with one persistent launch; use_graph replays a
+ * captured hipGraph of the launch.  Asynchronous. */
+int bx_batch_run(bx_batch *b, int32_t maxsteps, int32_t use_graph);
+
+/* Wait for the run; *ms receives its duration on the device (HIP events). */
+int bx_batch_sync(bx_batch *b, double *ms);
+
+/* Final rows (cons(5,.), u(3,.), p(.); any may be NULL), time, steps and
+ * status bits (1: the low-density EOS branch was met, 2: the vlimit clamp
+ * was met, 4: a non-finite dt stopped the tube) per tube. */
+int bx_batch_result(bx_batch *b, double *cons_rows, double *u_rows, double *p_rows, double *time, int32_t *steps,
+                    uint32_t *status);
+
+/* dt log, ntube x maxlog (entries past a tube's step count are unset). */
+int bx_batch_dts(bx_batch *b, double *dts);
+
 /* Synchronise the context's stream. */
 int bx_sync(bx_ctx *ctx);
 

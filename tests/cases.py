@@ -3,6 +3,7 @@ arrays, and drive any grid object through the main.f90:211-555 call sequence
 with snapshot hooks (tags match oracle/ref_harness.f90)."""
 from __future__ import annotations
 
+import dataclasses
 import hashlib
 from typing import Callable, Dict, Optional
 
@@ -43,6 +44,57 @@ def stress_ic(prm: HydroParams, seed: int):
     cons[fast, 4] = 1.0 + 0.5 * cons[fast, 0] * np.sum(u[fast] ** 2, axis=1)
     cons[cold, 4] = 0.25 * 0.5 * cons[cold, 0] * np.sum(u[cold] ** 2, axis=1)
     return ic
+
+
+def stress_tubes():
+    """Shock tubes that take the exact branches of the step: a vacuum-forming
+    rarefaction (state.f90:52-56 low-density branch, density and energy
+    floors), a clamped expansion (velocity.f90:33-50), gamma=5/3 with
+    non-cubic cells, a power-of-two spacing, a short tube and a tube with no
+    step to take (endtime 0)."""
+    from boxzyhydro_amd.ics import _riemann_params, cell_centres, tube_row
+
+    def tube(L, rl, rr, pl, pr, vl, vr, endtime=0.05, gamma=1.4, vlimit=1e3, dyz=None):
+        prm = _riemann_params(L, endtime, gamma, vlimit)
+        if dyz is not None:
+            prm = dataclasses.replace(prm, dy=dyz[0], dz=dyz[1])
+        x, _, _ = cell_centres(prm)
+        left = x < 0.0
+        return prm, tube_row(prm, np.where(left, rl, rr), np.where(left, vl, vr), np.where(left, pl, pr))
+
+    return [
+        tube(64, 0.1, 0.1, 1e-5, 1e-5, -5.0, 5.0, endtime=0.2),                # vacuum: rho < 1e-4
+        tube(80, 1.0, 1.0, 10.0, 0.01, 0.0, 0.0, vlimit=2.0),                   # clamp
+        tube(72, 2.0, 0.3, 3.0, 0.2, 0.3, -0.2, gamma=5.0 / 3.0, dyz=(0.02, 0.03)),
+        tube(64, 0.125, 1.0, 0.1, 1.0, 0.0, 0.0),                               # dx = 2^-6
+        tube(4, 1.0, 0.2, 1.0, 0.3, 0.5, 0.1, endtime=0.1),
+        tube(40, 1.0, 0.125, 1.0, 0.1, 0.0, 0.0, endtime=0.0),                  # no step
+    ]
+
+
+def oracle_tube(prm: HydroParams, row, threads: int = 4):
+    """Run the full grid of a tube row with the oracle; returns the dt and
+    time sequences, the final row (row iy=3, iz=1 of the grid; after a step
+    every row of the grid equals it with ghost cells holding their donors'
+    values, which is asserted) and the whole final grid."""
+    import pyoracle as po
+
+    from boxzyhydro_amd.batch import expand_row, grid_of_row
+
+    po.set_threads(threads)
+    og = po.OracleGrid(prm, grid_of_row(prm, row))
+    dts, times = og.run(1_000_000)
+    nx = prm.nx
+    fin = {"cons": og.cons.reshape(prm.shape + (5,))[0, 2].copy(), "u": og.u.reshape(prm.shape + (3,))[0, 2].copy(),
+           "p": og.p.reshape(prm.shape)[0, 2].copy()}
+    grid = {"cons": og.cons.copy(), "u": og.u.copy(), "p": og.p.copy()}
+    if len(dts):
+        donor = np.clip(np.arange(nx), 2, nx - 3)
+        fin = {k: v[donor].copy() for k, v in fin.items()}
+        full = expand_row(prm, fin)
+        for k in ("cons", "u", "p"):
+            assert np.array_equal(full[k].view(np.int64), grid[k].view(np.int64)), f"grid is not row-uniform ({k})"
+    return dts, times, fin, grid
 
 
 def case_inputs(rec: dict):
