@@ -8,10 +8,13 @@ stepped with the reference's full main-loop step (both RK stages, both
 half-step sources, the Courant reduction).  A step is one cell-update of every
 interior cell.  Inputs are resident in HBM before the timed region.
 
-Prints one JSON line with the cell-updates/s, the HBM-roofline of the dominant
-kernel measured with HIP events on the library's stream, and a CPU baseline:
-the reference's own Fortran subroutines (oracle/_ref) timed on the host on a
-bounded sample, or the C restatement of the oracle when that build is absent.
+Prints one JSON line: the FP64 cell-updates/s (value), the HBM roofline of
+the dominant kernel (the corrector's x/z kernel) timed with HIP events on the
+library's stream, the mixed-precision form on the same box, the batched
+shock tubes (512 problems, one persistent launch replayed from a hipGraph),
+the adjoint/forward cost (128^3 box, 200 steps, checkpoints every 20) and a
+CPU baseline: the reference's own Fortran subroutines (oracle/_ref) timed on
+the host on a bounded sample, or the C restatement when that build is absent.
 """
 from __future__ import annotations
 
@@ -24,9 +27,11 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
-B_ALG_FP64 = 200.0         # algorithmic bytes per cell-update (5 fields x 8 B x 5 streams)
-B_STAGE2_FP64 = 120.0      # stage-2 kernel: read U*, read U^n, write U^{n+1}
+HBM_PEAK = 8.0e12    # MI355X HBM3E, bytes/s
+B_STEP_FP64 = 200.0  # algorithmic bytes per cell-update: U^n in, U* out, U*, U^n in, U^{n+1} out (5 x 8 B each)
+B_STEP_MIXED = 100.0
+B_XZ2_FP64 = 120.0   # corrector x/z kernel: read U*, read U^n, write U^{n+1} (5 FP64 fields each)
+B_XZ2_MIXED = 60.0
 
 
 def log(msg: str) -> None:
@@ -61,7 +66,31 @@ def cpu_baseline(prm, ic, steps: int, threads: int):
                       f"step (oracle/boxzy_oracle.c)"}
 
 
-def mixed_study(prm, ic, steps, warmup, sync):
+def timed_run(prm, ic, precision, steps, warmup, tile=None):
+    """(cell-updates/s, ms per step, per-kernel average ms [y1, xz1, y2, xz2], x/z variant, fallbacks)."""
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+
+    g = HydroGrid(prm, ic, device=0, precision=precision)
+    if tile:
+        g.tune(*tile)
+    g.run(warmup, mode=nv.BX_STEP_FUSED)
+    g.sync()
+    g.profile(1)
+    t0 = time.perf_counter()
+    dts = g.run(steps, mode=nv.BX_STEP_FUSED)
+    g.sync()
+    elapsed = time.perf_counter() - t0
+    ms, nprof = g.profile(0)
+    if len(dts) != steps:
+        raise RuntimeError(f"took {len(dts)} of {steps} steps")
+    per = [m / max(nprof, 1) for m in ms]
+    fb = g.fallback_count()
+    del g
+    return prm.interior_cells * steps / elapsed, elapsed * 1e3 / steps, per, nprof, fb
+
+
+def mixed_study(prm, ic, steps):
     """The mixed form against the FP64 step on the same box: cell-updates/s of
     each and the per-field L1/Linf error of the mixed state after `steps`."""
     import numpy as np
@@ -96,6 +125,92 @@ def mixed_study(prm, ic, steps, warmup, sync):
     return res
 
 
+def batched(n_tubes: int, reps: int):
+    """512 shock tubes (L log-uniform in [64,4096], t_end=0.1) in one
+    persistent launch replayed from a captured hipGraph, work split picked by
+    the autotuner."""
+    import numpy as np
+
+    from boxzyhydro_amd import ics
+    from boxzyhydro_amd.autotune import tune_batch
+    from boxzyhydro_amd.batch import TubeBatch
+
+    tubes = ics.riemann_tubes(n_tubes, seed=0)
+    b = TubeBatch(tubes)
+    best, res = tune_batch(b, reps=1)
+    b.run(graph=True)
+    b.sync()
+    ms = min([(b.run(graph=True), b.sync())[1] for _ in range(reps)])
+    out = b.result()
+    steps = np.array([r["steps"] for r in out])
+    row_cells = np.asarray(b.nx - 4)
+    row_updates = float(np.sum(steps * row_cells))
+    _, interior3 = b.cells()
+    full_updates = float(np.sum(steps * row_cells * 16))  # each tube grid is L x 4 x 4 interior cells
+    return {"workload": f"{n_tubes} shock tubes, L log-uniform in [64,4096] (x4x4 interior), gamma 1.4, "
+                        f"t_end 0.1, FP64, one persistent launch replayed from a hipGraph",
+            "ms": ms, "steps_min": int(steps.min()), "steps_max": int(steps.max()),
+            "row_cell_updates_per_s": row_updates / (ms * 1e-3),
+            "grid_cell_updates_per_s": full_updates / (ms * 1e-3),
+            "tile": {"waves_per_block": best[0], "blocks_per_cu": best[1]},
+            "tile_ms": {f"{c[0]}x{c[1]}": t * 1e3 for c, t in res},
+            "longest_tube_steps": int(steps.max()),
+            "status_bits": int(np.bitwise_or.reduce([r["status"] for r in out]))}
+
+
+def adjoint_cost(n: int, nsteps: int, every: int):
+    """Backward (recompute from checkpoints + reverse sweep) against the plain
+    forward of the same frozen-dt map, on the adjoint workload."""
+    import numpy as np
+    import torch
+
+    from boxzyhydro_amd import ics
+    from boxzyhydro_amd.adjoint import AdjointStepper
+    from boxzyhydro_amd.params import box_params
+
+    prm = box_params(n)
+    ic = ics.box_ic(prm, seed=0)
+    tic = ics.box_ic(prm, seed=1)
+    dev = torch.device("cuda", 0)
+    st = AdjointStepper(prm, ic["u"], nsteps, every)
+    rho0 = torch.as_tensor(ic["cons"][:, 0].copy(), device=dev)
+    p0 = torch.as_tensor(ic["p"].copy(), device=dev)
+    tst = AdjointStepper(prm, tic["u"], nsteps, 0)
+    tgt = tst.forward(torch.as_tensor(tic["cons"][:, 0].copy(), device=dev),
+                      torch.as_tensor(tic["p"].copy(), device=dev), checkpoint=False)
+    del tst
+    interior = torch.as_tensor(ics.boundary_flags(prm).reshape(-1) == 0, device=dev)
+    t0 = time.perf_counter()
+    st.forward(rho0, p0)  # Courant dt of every step (read back per step), kept as the frozen sequence
+    t_courant = time.perf_counter() - t0
+    dts = st.dts.copy()
+    t_fwd = float("inf")
+    for _ in range(2):
+        t0 = time.perf_counter()
+        st.forward(rho0, p0, dts=dts, checkpoint=False)
+        t_fwd = min(t_fwd, time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    rT = st.forward(rho0, p0, dts=dts, checkpoint=True)
+    t_fwd_ck = time.perf_counter() - t0
+    lam = (rT - tgt) * interior
+    t_bwd = float("inf")
+    for _ in range(2):
+        t0 = time.perf_counter()
+        g_rho, g_p = st.backward(lam)
+        t_bwd = min(t_bwd, time.perf_counter() - t0)
+    return {"workload": f"box {n}^3 FP64, {nsteps} steps, checkpoints every {every}, frozen dt; "
+                        f"loss 0.5|rho_T - rho_target|^2, gradient w.r.t. initial rho and p",
+            "forward_ms": t_fwd * 1e3, "forward_courant_ms": t_courant * 1e3,
+            "forward_checkpointed_ms": t_fwd_ck * 1e3, "backward_ms": t_bwd * 1e3,
+            "adjoint_over_forward": t_bwd / t_fwd,
+            "adjoint_over_forward_courant": t_bwd / t_courant,
+            "note": "backward = recompute of each 20-step segment from its checkpoint + reverse sweep; "
+                    "forward_ms is the same frozen-dt map without checkpoints (no per-step read-back), "
+                    "forward_courant_ms the forward that also forms each step's Courant dt",
+            "grad_norm": float(torch.sqrt((g_rho * g_rho).sum() + (g_p * g_p).sum())),
+            "finite": bool(np.isfinite(float(g_rho.abs().max())))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,11 +218,15 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--mode", choices=["fused", "exact"], default="fused")
-    ap.add_argument("--tile", type=int, nargs=3, default=None, metavar=("YCHUNK", "ZCHUNK", "OCC"))
+    ap.add_argument("--tile", type=int, nargs=3, default=None, metavar=("YCHUNK", "ZCHUNK", "VARIANT"))
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision line")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched shock tubes")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost")
+    ap.add_argument("--adjoint-size", type=int, default=128)
+    ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--mixed-study", action="store_true",
                     help="mixed form vs FP64 at --size for --steps steps: speed and per-field error")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -136,41 +255,27 @@ def main() -> int:
     from boxzyhydro_amd.hydro import HydroGrid
     from boxzyhydro_amd.params import box_params
 
-    try:
-        import torch
-
-        sync = torch.cuda.synchronize
-    except Exception:  # torch is plumbing only
-        def sync():
-            return None
-
     prm = box_params(args.size)
     log(f"generating box {args.size}^3 ...")
     ic = ics.box_ic(prm, seed=0)
     if args.mixed_study:
-        print(json.dumps(mixed_study(prm, ic, args.steps, args.warmup, sync)))
+        print(json.dumps(mixed_study(prm, ic, args.steps)))
         return 0
-    mode = nv.BX_STEP_FUSED if args.mode == "fused" else nv.BX_STEP_EXACT
-    g = HydroGrid(prm, ic, device=0)
-    if args.tile:
-        g.tune(*args.tile)
-    log(f"warmup {args.warmup} steps ({args.mode}) ...")
-    g.run(args.warmup, mode=mode)
-    g.sync()
-    sync()
-    g.profile(1)
-    log(f"timing {args.steps} steps ...")
-    t0 = time.perf_counter()
-    dts = g.run(args.steps, mode=mode)
-    g.sync()
-    sync()
-    t1 = time.perf_counter()
-    s1_ms, s2_ms, nprof = g.profile(0)
-    elapsed = t1 - t0
-    if len(dts) != args.steps:
-        raise RuntimeError(f"took {len(dts)} of {args.steps} steps")
     cells = prm.interior_cells
-    value = cells * args.steps / elapsed
+    if args.mode == "exact":
+        g = HydroGrid(prm, ic, device=0)
+        g.run(args.warmup, mode=nv.BX_STEP_EXACT)
+        g.sync()
+        t0 = time.perf_counter()
+        g.run(args.steps, mode=nv.BX_STEP_EXACT)
+        g.sync()
+        el = time.perf_counter() - t0
+        print(json.dumps({"metric": "exact-sequence step", "value": cells * args.steps / el,
+                          "unit": "cell-updates/s", "ms_per_step": el * 1e3 / args.steps}))
+        return 0
+
+    log(f"FP64: warmup {args.warmup}, timing {args.steps} steps ...")
+    value, ms_step, per, nprof, fb = timed_run(prm, ic, nv.BX_PREC_FP64, args.steps, args.warmup, args.tile)
     out = {
         "metric": "cell-updates/s @256^3 FP64 vs mixed (HBM roofline %); adjoint/forward cost",
         "value": value,
@@ -178,7 +283,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -186,43 +291,38 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"3-D box {args.size}^3 interior FP64, outflow, gamma=5/3, CFL 0.5, "
                                f"seeded smoothed Gaussian rho/p, rms Mach 0.5; full main-loop step",
-                   "grid": [prm.nx, prm.ny, prm.nz], "mode": args.mode,
-                   "fallback_steps": g.fallback_count()},
-        "step_hbm_roofline_frac": value * B_ALG_FP64 / HBM_PEAK,
+                   "grid": [prm.nx, prm.ny, prm.nz], "mode": args.mode, "fallback_steps": fb},
+        "step_hbm_roofline_frac": value * B_STEP_FP64 / HBM_PEAK,
     }
-    if nprof > 0:
-        avg2 = s2_ms / nprof * 1e-3
-        avg1 = s1_ms / nprof * 1e-3
-        achieved = B_STAGE2_FP64 * cells / avg2
-        traffic = None
-        tf = os.path.join(ROOT, "profiles", f"traffic_stage2_box{args.size}.json")
-        if os.path.exists(tf):
-            with open(tf) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
-        out["roofline"] = {"kernel": "k_stage<2> (corrector stage)", "bound": "hbm", "achieved": achieved / 1e9,
-                           "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                           "traffic": traffic, "alg_bytes_per_launch": B_STAGE2_FP64 * cells,
-                           "avg_launch_ms": avg2 * 1e3, "stage1_avg_launch_ms": avg1 * 1e3}
-    else:
-        out["roofline"] = None
-    del g
+    xz2 = per[3] * 1e-3
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_k_xz2_box{args.size}.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    achieved = B_XZ2_FP64 * cells / xz2
+    out["roofline"] = {"kernel": "bx::k_xz<2, double, ...> (corrector stage, x/z faces + update)", "bound": "hbm",
+                       "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK, "traffic": traffic,
+                       "alg_bytes_per_launch": B_XZ2_FP64 * cells, "avg_launch_ms": per[3],
+                       "kernel_avg_ms": {"k_ydir<1>": per[0], "k_xz<1>": per[1], "k_ydir<2>": per[2],
+                                         "k_xz<2>": per[3]},
+                       "launches_timed": nprof}
     if not args.no_mixed:
         log(f"mixed form: {args.steps} steps ...")
-        gm = HydroGrid(prm, ic, device=0, precision=nv.BX_PREC_MIXED)
-        gm.run(args.warmup, mode=nv.BX_STEP_FUSED)
-        gm.sync()
-        gm.profile(1)
-        t0 = time.perf_counter()
-        gm.run(args.steps, mode=nv.BX_STEP_FUSED)
-        gm.sync()
-        tm = time.perf_counter() - t0
-        m1, m2, mn = gm.profile(0)
-        vm = cells * args.steps / tm
+        vm, msm, perm, _, fbm = timed_run(prm, ic, nv.BX_PREC_MIXED, args.steps, args.warmup)
         out["mixed"] = {"value": vm, "unit": "cell-updates/s", "dtype": "f32 storage+reconstruction, f64 update",
-                        "speedup_vs_fp64": vm / value, "ms_per_step": tm * 1e3 / args.steps,
-                        "hbm_roofline_frac": vm * 100.0 / HBM_PEAK,
-                        "stage_avg_launch_ms": [m1 / max(mn, 1), m2 / max(mn, 1)]}
-        del gm
+                        "speedup_vs_fp64": vm / value, "ms_per_step": msm,
+                        "step_hbm_roofline_frac": vm * B_STEP_MIXED / HBM_PEAK,
+                        "k_xz2_hbm_frac": B_XZ2_MIXED * cells / (perm[3] * 1e-3) / HBM_PEAK,
+                        "kernel_avg_ms": {"k_ydir<1>": perm[0], "k_xz<1>": perm[1], "k_ydir<2>": perm[2],
+                                          "k_xz<2>": perm[3]}, "fallback_steps": fbm}
+    if not args.no_batch:
+        log("batched shock tubes ...")
+        out["batched"] = batched(512, 3)
+    if not args.no_adjoint:
+        log(f"adjoint {args.adjoint_size}^3 x {args.adjoint_steps} ...")
+        out["adjoint"] = adjoint_cost(args.adjoint_size, args.adjoint_steps, 20)
     if not args.no_cpu:
         log(f"cpu baseline ({args.cpu_steps} steps, {args.cpu_threads} threads) ...")
         out["cpu_baseline"] = cpu_baseline(prm, ic, args.cpu_steps, args.cpu_threads)

@@ -32,3 +32,25 @@ def tune_step(grid, candidates: Iterable[Tuple[int, int, int]] = DEFAULT_CANDIDA
     best = min(results, key=lambda r: r[1])[0]
     grid.tune(*best)
     return best, results
+
+
+BATCH_CANDIDATES = [(16, 1), (16, 2), (8, 1), (8, 2), (4, 2), (4, 4)]
+
+
+def tune_batch(batch, candidates: Iterable[Tuple[int, int]] = BATCH_CANDIDATES, reps: int = 2,
+               graph: bool = True) -> Tuple[Tuple[int, int], List[Tuple[Tuple[int, int], float]]]:
+    """Work split of the batched-tube launch (waves per workgroup, workgroups
+    per compute unit): each candidate runs the whole batch ``reps`` times
+    from its set-up (device time of the launch, best of reps); the fastest is
+    kept.  Results do not depend on the choice."""
+    results = []
+    for cand in candidates:
+        batch.tune(*cand)
+        best = float("inf")
+        for _ in range(reps):
+            batch.run(graph=graph)
+            best = min(best, batch.sync())
+        results.append((cand, best * 1e-3))
+    best = min(results, key=lambda r: r[1])[0]
+    batch.tune(*best)
+    return best, results

@@ -309,13 +309,14 @@ int bx_destroy(bx_ctx *c)
     if (!c) return BX_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    adjoint_free(c);
     void *dev[] = {c->cons, c->cons_old, c->u, c->p, c->adindx, c->qq, c->states, c->ghost_dst, c->ghost_src,
                    c->ghost_sign, c->ghost_scratch_src, c->ghost_scratch, c->ustar, c->pstar, c->unext, c->dsc,
                    c->cons32, c->ustar32, c->unext32, c->pstar32, c->eflux};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (c->hsc) (void)hipHostFree(c->hsc);
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 6; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -672,7 +673,7 @@ static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, d
         const double dt = step_dt(rate, c->prm.cfl, endtime, *time, dtold, *step);
         if ((rc = fused_step(c, dt))) return rc;
         if ((rc = read_scalars(c))) return rc;
-        if (c->hsc->flags) {
+        if (c->hsc->flags & FLAG_IRREGULAR) {
             // an irregular branch: redo this step with the exact sequence from U^n
             c->fallbacks++;
             *time = time0;
@@ -692,11 +693,12 @@ static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, d
             continue;
         }
         if (c->prof_on) {
-            float t1 = 0.f, t2 = 0.f;
-            BX_HIP(hipEventElapsedTime(&t1, c->ev[0], c->ev[1]));
-            BX_HIP(hipEventElapsedTime(&t2, c->ev[2], c->ev[3]));
-            c->prof_ms[0] += t1;
-            c->prof_ms[1] += t2;
+            const int pairs[4][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}};
+            for (int q = 0; q < 4; q++) {
+                float t = 0.f;
+                BX_HIP(hipEventElapsedTime(&t, c->ev[pairs[q][0]], c->ev[pairs[q][1]]));
+                c->prof_ms[q] += t;
+            }
             c->prof_steps++;
         }
         if (c->state_in_32) std::swap(c->cons32, c->unext32);
@@ -760,20 +762,18 @@ extern "C" int bx_tune(bx_ctx *c, int32_t y_chunk, int32_t z_chunk, int32_t occu
     return BX_OK;
 }
 
-extern "C" int bx_profile(bx_ctx *c, int32_t enable, double *stage_ms, int64_t *steps)
+extern "C" int bx_profile(bx_ctx *c, int32_t enable, double *kernel_ms, int64_t *steps)
 {
     if (!c) return set_error(BX_ERR_ARG, "bx_profile: null context");
     BX_HIP(hipSetDevice(c->device));
     if (enable && !c->ev[0])
-        for (int i = 0; i < 4; i++) BX_HIP(hipEventCreate(&c->ev[i]));
-    if (stage_ms) {
-        stage_ms[0] = c->prof_ms[0];
-        stage_ms[1] = c->prof_ms[1];
-    }
+        for (int i = 0; i < 6; i++) BX_HIP(hipEventCreate(&c->ev[i]));
+    if (kernel_ms)
+        for (int q = 0; q < 4; q++) kernel_ms[q] = c->prof_ms[q];
     if (steps) *steps = c->prof_steps;
     if (enable >= 0) {
         c->prof_on = enable != 0;
-        c->prof_ms[0] = c->prof_ms[1] = 0.0;
+        for (int q = 0; q < 4; q++) c->prof_ms[q] = 0.0;
         c->prof_steps = 0;
     }
     return BX_OK;

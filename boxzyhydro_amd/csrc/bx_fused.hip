@@ -33,77 +33,6 @@
 
 namespace bx {
 
-// Raw values of one cell as stored (conserved variables, and p* in stage 2).
-template <typename S>
-struct Raw {
-    S r, m0, m1, m2, E, p;
-};
-
-template <int STAGE, typename S>
-__device__ __forceinline__ Raw<S> load_raw(const StageArgs &a, int64_t c)
-{
-    const int64_t n = a.n;
-    const S *uin = reinterpret_cast<const S *>(a.uin);
-    Raw<S> v;
-    v.r = uin[c];
-    v.m0 = uin[n + c];
-    v.m1 = uin[2 * n + c];
-    v.m2 = uin[3 * n + c];
-    v.E = uin[4 * n + c];
-    v.p = STAGE == 2 ? reinterpret_cast<const S *>(a.pin)[c] : S(0);
-    return v;
-}
-
-// primitive variables of a loaded cell for each stage.
-// FP64 storage: the reference's own operations.  FP32 storage (mixed form):
-// the kinetic energy, the energy/pressure subtraction and u are formed in FP64
-// from the stored values, then rounded for the FP32 reconstruction.
-template <int STAGE, typename S>
-__device__ __forceinline__ W6<S> prims(const StageArgs &a, const Raw<S> &v, unsigned int &flg)
-{
-    W6<S> w;
-    if constexpr (sizeof(S) == 8) {
-        const double r = v.r, m0 = v.m0, m1 = v.m1, m2 = v.m2, E = v.E;
-        w.d = r;
-        w.u0 = m0 / r;
-        w.u1 = m1 / r;
-        w.u2 = m2 / r;
-        if (STAGE == 1) {
-            // source(0) energy re-round with g = -0 (source.f90:176-182), then
-            // state at main.f90:314 with u = m/rho, velocity at :318
-            const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r;
-            const double E1 = vmax(E - ek, a.seps) + ek;
-            w.e = vmax(E1 - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
-            w.p = w.e * a.gm1;
-        } else {
-            // velocity at main.f90:355; p* is the state of :353 (stale velocity)
-            w.e = vmax(E - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
-            w.p = v.p;
-        }
-        if (fabs(w.u0) > a.vlimit || fabs(w.u1) > a.vlimit || fabs(w.u2) > a.vlimit) flg |= FLAG_CLAMP;
-    } else {
-        const double r = v.r, m0 = v.m0, m1 = v.m1, m2 = v.m2, E = v.E;
-        const double inv = 1.0 / r;
-        const double v0 = m0 * inv, v1 = m1 * inv, v2 = m2 * inv;
-        const double ek = 0.5 * (m0 * v0 + m1 * v1 + m2 * v2);
-        const double e = vmax(E - ek, a.seps);
-        w.d = (float)r;
-        w.u0 = (float)v0;
-        w.u1 = (float)v1;
-        w.u2 = (float)v2;
-        w.e = (float)e;
-        w.p = STAGE == 1 ? (float)(e * a.gm1) : v.p;
-        if (fabs(v0) > a.vlimit || fabs(v1) > a.vlimit || fabs(v2) > a.vlimit) flg |= FLAG_CLAMP;
-    }
-    return w;
-}
-
-template <int STAGE, typename S>
-__device__ __forceinline__ W6<S> load_prims(const StageArgs &a, int64_t c, unsigned int &flg)
-{
-    return prims<STAGE, S>(a, load_raw<STAGE, S>(a, c), flg);
-}
-
 // Per-wave reduction of a non-negative max into *dst; the atomic is skipped
 // when the stored maximum is already larger (almost always, after warm-up).
 __device__ __forceinline__ void wave_max_to(double v, unsigned long long *dst)
@@ -126,7 +55,7 @@ template <int STAGE, typename S>
 __global__ __launch_bounds__(256) void k_ydir(StageArgs a, S *__restrict__ Ey, int ly)
 {
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    const int w = wave_id();
     const int nx = a.nx, ny = a.ny;
     const int64_t n = a.n, plane = (int64_t)nx * ny;
     const int x = 2 + blockIdx.x * 64 + lane;
@@ -138,15 +67,16 @@ __global__ __launch_bounds__(256) void k_ydir(StageArgs a, S *__restrict__ Ey, i
     const int je = min(jb + ly, ny - 2);
     if (jb >= je) return;
     const bool zg = !a.flatz && (z < 2 || z > a.nz - 3);
-    const int64_t base = (int64_t)z * plane + xc;
-    auto cell = [&](int j) -> int64_t { return base + (int64_t)j * nx; };
+    const int64_t zbase = (int64_t)z * plane;
+    const unsigned xoff = (unsigned)xc * (unsigned)sizeof(S);
+    auto cell = [&](int j) -> int64_t { return zbase + (int64_t)j * nx; };  // uniform part of the cell offset
     auto yghost = [&](int j) -> bool { return zg || j < 2 || j > ny - 3; };
     unsigned int flg = 0;
 
-    W6<S> q0 = load_prims<STAGE, S>(a, cell(jb - 2), flg);
-    W6<S> q1 = load_prims<STAGE, S>(a, cell(jb - 1), flg);
-    W6<S> q2 = load_prims<STAGE, S>(a, cell(jb), flg);
-    W6<S> q3 = load_prims<STAGE, S>(a, cell(jb + 1), flg);
+    W6<S> q0 = load_prims<STAGE, S>(a, cell(jb - 2), xoff, flg);
+    W6<S> q1 = load_prims<STAGE, S>(a, cell(jb - 1), xoff, flg);
+    W6<S> q2 = load_prims<STAGE, S>(a, cell(jb), xoff, flg);
+    W6<S> q3 = load_prims<STAGE, S>(a, cell(jb + 1), xoff, flg);
     Side<S> mi, pl, yp;
     dir_states<1>(q0, q1, q2, sound_of(q1, a.gam), yghost(jb - 1), a, mi, pl);
     yp = pl;
@@ -156,17 +86,17 @@ __global__ __launch_bounds__(256) void k_ydir(StageArgs a, S *__restrict__ Ey, i
     yp = pl;
     q0 = q2;
     q1 = q3;
-    Raw<S> nxt = load_raw<STAGE, S>(a, cell(jb + 2));
+    Raw<S> nxt = load_raw<STAGE, S>(a, cell(jb + 2), xoff);
     for (int j = jb; j < je; j++) {
         const Raw<S> cur = nxt;
-        nxt = load_raw<STAGE, S>(a, cell(j + 3));  // row j+3 <= ny-1: prefetch for the next row
+        nxt = load_raw<STAGE, S>(a, cell(j + 3), xoff);  // row j+3 <= ny-1: prefetch for the next row
         q2 = prims<STAGE, S>(a, cur, flg);
         dir_states<1>(q0, q1, q2, sound_of(q1, a.gam), yghost(j + 1), a, mi, pl);
         S F1[5];
         face_flux<S, 1>(yp, mi, F1);
         if (xin) {
             const int64_t c = cell(j);
-            for (int i = 0; i < 5; i++) Ey[(int64_t)i * n + c] = (S)(a.axz * ((double)F1[i] - (double)F0[i]));
+            for (int i = 0; i < 5; i++) elem<S>(Ey, (int64_t)i * n + c, xoff) = (S)(a.axz * ((double)F1[i] - (double)F0[i]));
         }
         for (int i = 0; i < 5; i++) F0[i] = F1[i];
         yp = pl;
@@ -188,18 +118,16 @@ template <int STAGE, typename S, int OCC, bool PFE>
 __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const S *__restrict__ Ey)
 {
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
+    const int w = wave_id();
     const int nx = a.nx;
     const int64_t n = a.n, plane = (int64_t)nx * a.ny;
-    const S *uin = reinterpret_cast<const S *>(a.uin);
-    S *uout = reinterpret_cast<S *>(a.uout);
 
     const int64_t F0 = 2 * (int64_t)nx - 2 + (int64_t)(blockIdx.x * 4 + w) * kSegOut;  // lane 0
     if (F0 + 2 >= (int64_t)(a.ny - 2) * nx) return;  // whole wave past the interior rows
     int64_t F = F0 + lane;
     if (F >= plane) F = plane - 1;  // halo lanes of the last wave: clamp for the loads
     const int y = (int)(F / nx), x = (int)(F - (int64_t)y * nx);
-    const int64_t col = F;
+    const unsigned coff = (unsigned)F * (unsigned)sizeof(S);  // per-lane byte offset within a z-plane
     const bool x_ghost = (x < 2) || (x > nx - 3) || (y < 2) || (y > a.ny - 3);
     const bool is_out = lane >= 2 && lane < 62 && !x_ghost;
 
@@ -220,10 +148,10 @@ __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const S *__restric
     Side<S> zp_prev, zmi, zpl;
     S Fz_prev[5] = {S(0), S(0), S(0), S(0), S(0)};
     if (!a.flatz) {
-        q0 = load_prims<STAGE, S>(a, zplane(kb - 2) + col, flg);
-        q1 = load_prims<STAGE, S>(a, zplane(kb - 1) + col, flg);
-        q2 = load_prims<STAGE, S>(a, zplane(kb) + col, flg);
-        const W6<S> q3 = load_prims<STAGE, S>(a, zplane(kb + 1) + col, flg);
+        q0 = load_prims<STAGE, S>(a, zplane(kb - 2), coff, flg);
+        q1 = load_prims<STAGE, S>(a, zplane(kb - 1), coff, flg);
+        q2 = load_prims<STAGE, S>(a, zplane(kb), coff, flg);
+        const W6<S> q3 = load_prims<STAGE, S>(a, zplane(kb + 1), coff, flg);
         dir_states<2>(q0, q1, q2, sound_of(q1, a.gam), x_ghost || zghost(kb - 1), a, zmi, zpl);
         zp_prev = zpl;
         cs0 = sound_of(q2, a.gam);
@@ -233,24 +161,24 @@ __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const S *__restric
         q0 = q2;
         q1 = q3;
     } else {
-        q0 = load_prims<STAGE, S>(a, zplane(kb) + col, flg);
+        q0 = load_prims<STAGE, S>(a, zplane(kb), coff, flg);
         q1 = q0;
         zp_prev.d = zp_prev.p = zp_prev.e = zp_prev.g = S(0);
         zp_prev.v[0] = zp_prev.v[1] = zp_prev.v[2] = S(0);
     }
 
     auto lead = [&](int k) -> int { return a.flatz ? min(k + 1, ke - 1) : min(k + 2, a.nz - 1); };
-    Raw<S> nxt = load_raw<STAGE, S>(a, zplane(lead(kb)) + col);
+    Raw<S> nxt = load_raw<STAGE, S>(a, zplane(lead(kb)), coff);
     for (int k = kb; k < ke; k++) {
         const Raw<S> cur = nxt;
-        nxt = load_raw<STAGE, S>(a, zplane(lead(k + 1)) + col);  // prefetch the next leading plane
+        nxt = load_raw<STAGE, S>(a, zplane(lead(k + 1)), coff);  // prefetch the next leading plane
         q2 = prims<STAGE, S>(a, cur, flg);
         const bool ghost_k = x_ghost || zghost(k);
         // the y-term of this plane (written by k_ydir, not yet in cache): load early
-        const int64_t cu = zplane(k) + col;
+        const int64_t cu = zplane(k);
         double ey[5];
         if (PFE)
-            for (int i = 0; i < 5; i++) ey[i] = Ey[(int64_t)i * n + cu];
+            for (int i = 0; i < 5; i++) ey[i] = elem<S>(Ey, (int64_t)i * n + cu, coff);
 
         // ---- z face k+1/2 from the register queue
         S Fz1[5] = {S(0), S(0), S(0), S(0), S(0)};
@@ -292,14 +220,14 @@ __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const S *__restric
         }
 
         if (is_out) {
-            const int64_t c = zplane(k) + col;
+            const int64_t c = zplane(k);
             double ft[5];
             if (!PFE)
-                for (int i = 0; i < 5; i++) ey[i] = Ey[(int64_t)i * n + c];
+                for (int i = 0; i < 5; i++) ey[i] = elem<S>(Ey, (int64_t)i * n + c, coff);
             double ui[5], uo[5];
             for (int i = 0; i < 5; i++) {
-                ui[i] = uin[(int64_t)i * n + c];
-                uo[i] = STAGE == 2 ? (double)reinterpret_cast<const S *>(a.uold)[(int64_t)i * n + c] : 0.0;
+                ui[i] = elem<S>(a.uin, (int64_t)i * n + c, coff);
+                uo[i] = STAGE == 2 ? (double)elem<S>(a.uold, (int64_t)i * n + c, coff) : 0.0;
             }
             for (int i = 0; i < 5; i++) {
                 // flux.f90:188-190 in the reference's order, accumulated in FP64
@@ -327,6 +255,7 @@ __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const S *__restric
                 for (int i = 0; i < 5; i++) U[i] = 0.5 * (old[i] + ui[i] + ft[i]);
             }
             // floors, flux.f90:225-228
+            if (!(U[0] > a.srho) || U[4] < 0.0) flg |= FLAG_FLOOR;
             U[0] = vmax(U[0], a.srho);
             if (U[4] < 0.0) U[4] = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0] + a.seps;
             if (STAGE == 1) {
@@ -337,19 +266,21 @@ __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const S *__restric
                 const double r1 = Us[0], E1 = Us[4];
                 const double w0 = q0.u0, w1 = q0.u1, w2 = q0.u2;
                 const double ekin = r1 * 0.5 * (w0 * w0 + w1 * w1 + w2 * w2);
-                reinterpret_cast<S *>(a.pout)[c] = (S)(vmax(E1 - ekin, a.seps) * a.gm1);
-                for (int i = 0; i < 5; i++) uout[(int64_t)i * n + c] = Us[i];
+                if (!(E1 - ekin > a.seps)) flg |= FLAG_FLOOR;
+                elem<S>(a.pout, c, coff) = (S)(vmax(E1 - ekin, a.seps) * a.gm1);
+                for (int i = 0; i < 5; i++) elem<S>(a.uout, (int64_t)i * n + c, coff) = Us[i];
             } else {
                 // main.f90:374-515: state, velocity, source(1), state, velocity
                 if (U[0] < 1e-4) flg |= FLAG_LOWRHO;
                 const double v0 = U[1] / U[0], v1 = U[2] / U[0], v2 = U[3] / U[0];
                 if (fabs(v0) > a.vlimit || fabs(v1) > a.vlimit || fabs(v2) > a.vlimit) flg |= FLAG_CLAMP;
                 const double ek = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0];
+                if (!(U[4] - ek > a.seps)) flg |= FLAG_FLOOR;
                 U[4] = vmax(U[4] - ek, a.seps) + ek;
                 S Us[5];
                 for (int i = 0; i < 5; i++) {
                     Us[i] = (S)U[i];
-                    uout[(int64_t)i * n + c] = Us[i];
+                    elem<S>(a.uout, (int64_t)i * n + c, coff) = Us[i];
                 }
                 // next step: state (main.f90:232) and courant rate (courant.f90:20-21)
                 double p, w0, w1, w2, rr;
@@ -476,7 +407,7 @@ static bool is_pow2(double d)
     return m == 0.5;
 }
 
-static StageArgs base_args(bx_ctx *c, double dt)
+StageArgs base_args(bx_ctx *c, double dt)
 {
     StageArgs a{};
     a.n = c->n;
@@ -511,7 +442,7 @@ static StageArgs base_args(bx_ctx *c, double dt)
 }
 
 template <int STAGE, typename S>
-static int launch_stage(bx_ctx *c, const StageArgs &a)
+static int launch_stage(bx_ctx *c, const StageArgs &a, hipEvent_t mid)
 {
     const int nzi = a.zhi - a.zlo;
     const int ly = c->fused_ly;
@@ -522,6 +453,7 @@ static int launch_stage(bx_ctx *c, const StageArgs &a)
         hipLaunchKernelGGL((k_ydir<STAGE, S>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, (S *)c->eflux, ly);
         BX_HIP(hipGetLastError());
     }
+    if (mid) BX_HIP(hipEventRecord(mid, c->stream));
     {
         const int64_t nwaves = ((int64_t)(c->ny - 4) * c->nx + kSegOut - 1) / kSegOut;
         const unsigned gx = (unsigned)((nwaves + 3) / 4);
@@ -568,9 +500,22 @@ int fused_prepare(bx_ctx *c)
     if (!c->pstar) BX_HIP(hipMalloc((void **)&c->pstar, pb));
     // defaults from the autotuner on the 256^3 box (FP64: 64 rows, 32 planes;
     // mixed: 64 rows, 16 planes, y-term loaded early)
-    if (c->fused_ly == 0) c->fused_ly = 64;
+    // default work split (autotuned on the 256^3 box: FP64 64 rows and 32 planes
+    // per wave, mixed 16 planes with the y-term loaded early), shortened on
+    // smaller grids until the y kernel has about 2 and the x/z kernel about 4
+    // waves per SIMD to run
+    const int64_t want = 2 * 1024;
+    const int nzi = c->flatz ? c->nz : c->nz - 4;
+    if (c->fused_ly == 0) {
+        int ly = 64;
+        while (ly > 8 && (int64_t)((c->nx - 4 + 63) / 64) * nzi * ((c->ny - 4 + ly - 1) / ly) < want) ly /= 2;
+        c->fused_ly = ly;
+    }
     if (c->fused_lz == 0) {
-        c->fused_lz = c->prm.precision == BX_PREC_MIXED ? 16 : 32;
+        int lz = c->prm.precision == BX_PREC_MIXED ? 16 : 32;
+        const int64_t rowwaves = ((int64_t)(c->ny - 4) * c->nx + kSegOut - 1) / kSegOut;
+        while (lz > 4 && rowwaves * ((nzi + lz - 1) / lz) < 2 * want) lz /= 2;
+        c->fused_lz = lz;
         c->fused_occ = c->prm.precision == BX_PREC_MIXED ? 1 : 0;
     }
     return BX_OK;
@@ -620,18 +565,18 @@ int widen_cons(bx_ctx *c)
 // accumulates into rate_bits[0] and irregular branches into flags (both
 // zeroed here).
 template <typename S>
-static int fused_step_t(bx_ctx *c, double dt, S *u0, S *us, S *ps, S *un)
+static int fused_step_t(bx_ctx *c, double dt, S *u0, S *us, S *ps, S *un, bool reset_flags = true)
 {
     int rc;
-    BX_HIP(hipMemsetAsync(&c->dsc->flags, 0, sizeof(unsigned int), c->stream));
+    if (reset_flags) BX_HIP(hipMemsetAsync(&c->dsc->flags, 0, sizeof(unsigned int), c->stream));
     BX_HIP(hipMemsetAsync(&c->dsc->rate_bits[0], 0, sizeof(unsigned long long), c->stream));
     StageArgs a = base_args(c, dt);
     a.uin = u0;
     a.uout = us;
     a.pout = ps;
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[0], c->stream));
-    if ((rc = launch_stage<1, S>(c, a))) return rc;
-    if (c->prof_on) BX_HIP(hipEventRecord(c->ev[1], c->stream));
+    if ((rc = launch_stage<1, S>(c, a, c->prof_on ? c->ev[1] : nullptr))) return rc;
+    if (c->prof_on) BX_HIP(hipEventRecord(c->ev[2], c->stream));
     if ((rc = ghosts6<S>(c, us, ps))) return rc;
     StageArgs b = base_args(c, dt);
     b.uin = us;
@@ -639,11 +584,16 @@ static int fused_step_t(bx_ctx *c, double dt, S *u0, S *us, S *ps, S *un)
     b.uold = u0;
     b.uout = un;
     b.rate = &c->dsc->rate_bits[0];
-    if (c->prof_on) BX_HIP(hipEventRecord(c->ev[2], c->stream));
-    if ((rc = launch_stage<2, S>(c, b))) return rc;
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[3], c->stream));
+    if ((rc = launch_stage<2, S>(c, b, c->prof_on ? c->ev[4] : nullptr))) return rc;
+    if (c->prof_on) BX_HIP(hipEventRecord(c->ev[5], c->stream));
     if ((rc = ghosts6<S>(c, un, (S *)nullptr))) return rc;
     return BX_OK;
+}
+
+int fused_step_into(bx_ctx *c, double dt, double *u0, double *us, double *ps, double *un, bool reset_flags)
+{
+    return fused_step_t<double>(c, dt, u0, us, ps, un, reset_flags);
 }
 
 int fused_step(bx_ctx *c, double dt)

@@ -29,6 +29,9 @@ struct DevScalars {
 enum : unsigned int {
     FLAG_LOWRHO = 1u,  // a state() call met rho<1e-4 (state.f90:52-56)
     FLAG_CLAMP = 2u,   // a velocity() call met |u_i|>vlimit (velocity.f90:33)
+    FLAG_FLOOR = 8u,   // a floor of the stage update acted (flux.f90:225-228, the source re-round
+                       // or the p* eps floor); informational: the step is exact either way
+    FLAG_IRREGULAR = FLAG_LOWRHO | FLAG_CLAMP,
 };
 
 }  // namespace bx
@@ -75,8 +78,8 @@ struct bx_ctx {
     int fused_occ;           // x/z kernel built for >= 4 waves/SIMD (register cap) when 4
     double *eflux;           // 5 planes: y-term of fluxtmp (FP64)
     bool prof_on;
-    hipEvent_t ev[4];
-    double prof_ms[2];
+    hipEvent_t ev[6];   // per stage: before the y kernel, between the two kernels, after the x/z kernel
+    double prof_ms[4];  // summed durations: y kernel stage 1, x/z kernel stage 1, y kernel stage 2, x/z kernel stage 2
     int64_t prof_steps;
     bool aux_stale;    // u, p planes not materialised since the last fused step
     bool old_stale;    // cons_old not materialised since the last fused step
@@ -88,6 +91,8 @@ struct bx_ctx {
     // mixed-precision planes (FP32 storage)
     float *cons32, *ustar32, *unext32, *pstar32;
     bool state_in_32;  // the current state lives in cons32 (mixed form)
+
+    struct bx_adj *adj;  // adjoint store (bx_adjoint.hip), or null
 };
 
 namespace bx {
@@ -117,6 +122,10 @@ int launch_avisc(bx_ctx *c);
 int launch_materialize(bx_ctx *c);
 int launch_regular_check(bx_ctx *c);
 int fused_step(bx_ctx *c, double dt);
+// one FP64 fused step U^n=u0 -> U*=us, p*=ps -> U^{n+1}=un (ghosts filled); the rate is zeroed
+// first, the flags too when reset_flags (else they accumulate over steps)
+int fused_step_into(bx_ctx *c, double dt, double *u0, double *us, double *ps, double *un, bool reset_flags);
+void adjoint_free(bx_ctx *c);
 int fused_prepare(bx_ctx *c);
 int narrow_cons(bx_ctx *c);
 int widen_cons(bx_ctx *c);

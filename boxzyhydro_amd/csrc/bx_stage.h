@@ -12,6 +12,19 @@ namespace bx {
 
 constexpr int kSegOut = 60;  // output cells per wave row
 
+// Element of an array addressed as a wave-uniform element offset `u` from the
+// base plus a per-lane 32-bit byte offset: the compiler then forms one uniform
+// 64-bit base per plane (scalar registers) and keeps a single offset register
+// per lane for every plane, instead of a 64-bit vector address per plane.
+template <typename S>
+__device__ __forceinline__ S &elem(const void *base, int64_t u, unsigned boff)
+{
+    return *reinterpret_cast<S *>(reinterpret_cast<char *>(const_cast<void *>(base)) + u * (int64_t)sizeof(S) + boff);
+}
+
+// the wave index of a thread, marked wave-uniform for the compiler
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
 
 struct StageArgs {
     int64_t n;
@@ -32,6 +45,9 @@ struct StageArgs {
     unsigned int *flags;
     unsigned long long *rate;  // stage 2: next step's Courant rate (max)
 };
+
+// Stage arguments of a context for a step of full dt (bx_fused.hip)
+StageArgs base_args(bx_ctx *c, double dt);
 
 template <typename T>
 struct W6 {
@@ -111,6 +127,78 @@ template <typename T>
 __device__ __forceinline__ T sound_of(const W6<T> &w, double gam)
 {
     return Arith<T>::root(Arith<T>::div(w.p * T(gam), w.d));
+}
+
+
+// Raw values of one cell as stored (conserved variables, and p* in stage 2).
+template <typename S>
+struct Raw {
+    S r, m0, m1, m2, E, p;
+};
+
+// cell = uniform element offset u + per-lane byte offset boff
+template <int STAGE, typename S>
+__device__ __forceinline__ Raw<S> load_raw(const StageArgs &a, int64_t u, unsigned boff)
+{
+    const int64_t n = a.n;
+    Raw<S> v;
+    v.r = elem<S>(a.uin, u, boff);
+    v.m0 = elem<S>(a.uin, n + u, boff);
+    v.m1 = elem<S>(a.uin, 2 * n + u, boff);
+    v.m2 = elem<S>(a.uin, 3 * n + u, boff);
+    v.E = elem<S>(a.uin, 4 * n + u, boff);
+    v.p = STAGE == 2 ? elem<S>(a.pin, u, boff) : S(0);
+    return v;
+}
+
+// primitive variables of a loaded cell for each stage.
+// FP64 storage: the reference's own operations.  FP32 storage (mixed form):
+// the kinetic energy, the energy/pressure subtraction and u are formed in FP64
+// from the stored values, then rounded for the FP32 reconstruction.
+template <int STAGE, typename S>
+__device__ __forceinline__ W6<S> prims(const StageArgs &a, const Raw<S> &v, unsigned int &flg)
+{
+    W6<S> w;
+    if constexpr (sizeof(S) == 8) {
+        const double r = v.r, m0 = v.m0, m1 = v.m1, m2 = v.m2, E = v.E;
+        w.d = r;
+        w.u0 = m0 / r;
+        w.u1 = m1 / r;
+        w.u2 = m2 / r;
+        if (STAGE == 1) {
+            // source(0) energy re-round with g = -0 (source.f90:176-182), then
+            // state at main.f90:314 with u = m/rho, velocity at :318
+            const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r;
+            const double E1 = vmax(E - ek, a.seps) + ek;
+            w.e = vmax(E1 - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
+            w.p = w.e * a.gm1;
+        } else {
+            // velocity at main.f90:355; p* is the state of :353 (stale velocity)
+            w.e = vmax(E - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
+            w.p = v.p;
+        }
+        if (fabs(w.u0) > a.vlimit || fabs(w.u1) > a.vlimit || fabs(w.u2) > a.vlimit) flg |= FLAG_CLAMP;
+    } else {
+        const double r = v.r, m0 = v.m0, m1 = v.m1, m2 = v.m2, E = v.E;
+        const double inv = 1.0 / r;
+        const double v0 = m0 * inv, v1 = m1 * inv, v2 = m2 * inv;
+        const double ek = 0.5 * (m0 * v0 + m1 * v1 + m2 * v2);
+        const double e = vmax(E - ek, a.seps);
+        w.d = (float)r;
+        w.u0 = (float)v0;
+        w.u1 = (float)v1;
+        w.u2 = (float)v2;
+        w.e = (float)e;
+        w.p = STAGE == 1 ? (float)(e * a.gm1) : v.p;
+        if (fabs(v0) > a.vlimit || fabs(v1) > a.vlimit || fabs(v2) > a.vlimit) flg |= FLAG_CLAMP;
+    }
+    return w;
+}
+
+template <int STAGE, typename S>
+__device__ __forceinline__ W6<S> load_prims(const StageArgs &a, int64_t u, unsigned boff, unsigned int &flg)
+{
+    return prims<STAGE, S>(a, load_raw<STAGE, S>(a, u, boff), flg);
 }
 
 

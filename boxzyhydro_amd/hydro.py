@@ -167,11 +167,11 @@ class HydroGrid:
 
     def profile(self, enable: int = -1):
         """Event timing of the fused stage kernels: enable=1 start, 0 stop,
-        -1 read.  Returns (stage1_ms_total, stage2_ms_total, steps)."""
-        ms = (ctypes.c_double * 2)()
+        -1 read.  Returns ([y1_ms, xz1_ms, y2_ms, xz2_ms] summed, steps)."""
+        ms = (ctypes.c_double * 4)()
         steps = ctypes.c_int64()
         nv.check(nv.lib().bx_profile(self._h, enable, ms, ctypes.byref(steps)), "bx_profile")
-        return ms[0], ms[1], steps.value
+        return list(ms), steps.value
 
     def fallback_count(self) -> int:
         c = ctypes.c_int64()

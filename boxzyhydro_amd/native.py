@@ -7,7 +7,9 @@ is no CPU fallback on the product path.
 from __future__ import annotations
 
 import ctypes
+import importlib.util
 import os
+import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libboxzy_hip.so")
@@ -80,6 +82,10 @@ _SIGS = {
     "bx_batch_sync": [_VP, _DP],
     "bx_batch_result": [_VP, _DP, _DP, _DP, _DP, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_uint32)],
     "bx_batch_dts": [_VP, _DP],
+    "bx_adjoint_forward": [_VP, _I32, _I32, _DP, _DP],
+    "bx_adjoint_backward": [_VP, _VP, _VP],
+    "bx_adjoint_tune": [_VP, _I32, _I32],
+    "bx_cons_device": [_VP, _VP, _I32],
 }
 
 EXPORTED = tuple(_SIGS) + ("bx_last_error",)
@@ -89,6 +95,10 @@ def lib() -> ctypes.CDLL:
     """Load the HIP library (raises if it has not been built)."""
     global _lib
     if _lib is None:
+        # torch ships its own HIP runtime libraries; when both are used in one
+        # process they work if torch's are loaded first, so load them now
+        if "torch" not in sys.modules and importlib.util.find_spec("torch") is not None:
+            import torch  # noqa: F401
         if not os.path.exists(LIB_PATH):
             raise BoxzyError(f"{LIB_PATH} is missing: build it with `python -m boxzyhydro_amd.build`")
         L = ctypes.CDLL(LIB_PATH)

@@ -110,9 +110,10 @@ int bx_fallback_count(const bx_ctx *ctx, int64_t *count);
 int bx_tune(bx_ctx *ctx, int32_t y_chunk, int32_t z_chunk, int32_t variant);
 
 /* Event timing of the fused stage kernels on the context's stream.
- * enable=1 starts (and clears), 0 stops, -1 only reads.  stage_ms[2]
- * receives the summed durations of stage 1 and stage 2 over *steps steps. */
-int bx_profile(bx_ctx *ctx, int32_t enable, double *stage_ms, int64_t *steps);
+ * enable=1 starts (and clears), 0 stops, -1 only reads.  kernel_ms[4]
+ * receives the summed durations of the y-direction and the x/z kernel of
+ * stage 1, then of stage 2, over *steps steps. */
+int bx_profile(bx_ctx *ctx, int32_t enable, double *kernel_ms, int64_t *steps);
 
 /* ------------------------------------------------------------------------
  * Batched shock tubes.  The reference runs one grid per process; this entry
@@ -173,6 +174,38 @@ int bx_batch_result(bx_batch *b, double *cons_rows, double *u_rows, double *p_ro
 
 /* dt log, ntube x maxlog (entries past a tube's step count are unset). */
 int bx_batch_dts(bx_batch *b, double *dts);
+
+/* ------------------------------------------------------------------------
+ * Adjoint of the FP64 step with a frozen dt sequence (no reference
+ * counterpart: the reference has no derivatives).  The differentiated map is
+ * the step of main.f90:211-555 on a regular state (ghost cells copies of their
+ * donors, u = m/rho, rho >= 1e-4, |u| <= vlimit), as bx_step(BX_STEP_FUSED)
+ * computes it, with dt given per step.  Branches are differentiated as taken
+ * (limiter argument, HLL speed bounds, MUSCL lambda, internal-energy floors);
+ * a run in which a floor of the stage update acts is refused.
+ *
+ * bx_adjoint_forward: fill the ghost cells of the current state, then take
+ * nsteps steps, with dts_in[k] when dts_in is given, else with the Courant dt
+ * of main.f90:266-281 (no endtime clamp); dts_out (may be NULL) receives the
+ * dt of every step.  every > 0 keeps a checkpoint of the state every `every`
+ * steps for bx_adjoint_backward; every == 0 keeps none.
+ *
+ * bx_adjoint_backward: lam_T and lam_0 are DEVICE pointers to 5 SoA planes of
+ * ngrid doubles (rho, mx, my, mz, E): the adjoint of the final state of the
+ * last checkpointed forward, and the resulting adjoint of its initial state
+ * (ghost-cell adjoints folded into their donors).  Each segment is recomputed
+ * from its checkpoint before its reverse sweep.
+ */
+int bx_adjoint_forward(bx_ctx *ctx, int32_t nsteps, int32_t every, const double *dts_in, double *dts_out);
+int bx_adjoint_backward(bx_ctx *ctx, const double *lam_T, double *lam_0);
+
+/* Chunk lengths (cells) of the y- and z-direction reverse kernels; results do not change. */
+int bx_adjoint_tune(bx_ctx *ctx, int32_t y_chunk, int32_t z_chunk);
+
+/* Copy the FP64 conserved state between the context and a DEVICE buffer of
+ * 5 SoA planes of ngrid doubles: to_ctx=1 sets the state (u and p are
+ * recomputed by the next step), 0 reads it.  Synchronises. */
+int bx_cons_device(bx_ctx *ctx, double *planes, int32_t to_ctx);
 
 /* Synchronise the context's stream. */
 int bx_sync(bx_ctx *ctx);

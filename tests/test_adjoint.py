@@ -1,0 +1,146 @@
+"""Adjoint of the FP64 step (bx_adjoint_*, boxzyhydro_amd.adjoint) against
+central finite differences of the GPU forward with the same frozen dt
+sequence: per-cell gradient entries and directional derivatives (the
+forward's tangent), plus determinism and the autograd wiring."""
+import numpy as np
+import pytest
+
+from boxzyhydro_amd import ics
+from boxzyhydro_amd.params import box_params
+
+EPS = 1e-6
+
+
+def _setup(n=16, nsteps=8, every=3, seed=0):
+    import torch
+
+    from boxzyhydro_amd.adjoint import AdjointStepper
+
+    prm = box_params(n)
+    ic = ics.box_ic(prm, seed=seed)
+    tic = ics.box_ic(prm, seed=seed + 1)
+    dev = torch.device("cuda", 0)
+    st = AdjointStepper(prm, ic["u"], nsteps, every)
+    rho0 = torch.as_tensor(ic["cons"][:, 0].copy(), device=dev)
+    p0 = torch.as_tensor(ic["p"].copy(), device=dev)
+    # target: the second seed evolved forward with its own Courant dt
+    tst = AdjointStepper(prm, tic["u"], nsteps, 0)
+    tgt = tst.forward(torch.as_tensor(tic["cons"][:, 0].copy(), device=dev),
+                      torch.as_tensor(tic["p"].copy(), device=dev), checkpoint=False)
+    interior = torch.as_tensor(ics.boundary_flags(prm).reshape(-1) == 0, device=dev)
+    st.forward(rho0, p0)  # fixes the dt sequence
+    return prm, st, rho0, p0, tgt, interior
+
+
+def _final(st, rho, p):
+    return st.forward(rho, p, dts=st.dts, checkpoint=False)
+
+
+def _dloss(a, b, tgt, interior):
+    """L(a) - L(b) for L = |rho_T - tgt|^2/2 over interior cells, term by term
+    ((a-b)(a+b-2 tgt)/2: the difference of the final densities is formed
+    before any cancellation)."""
+    return float((0.5 * (a - b) * (a + b - 2 * tgt) * interior).sum())
+
+
+def _central(st, rho0, p0, d_rho, d_p, eps, tgt, interior):
+    a = _final(st, rho0 + eps * d_rho, p0 + eps * d_p)
+    b = _final(st, rho0 - eps * d_rho, p0 - eps * d_p)
+    base = _final(st, rho0, p0)
+    return (_dloss(a, b, tgt, interior) / (2 * eps), _dloss(a, base, tgt, interior) / eps,
+            _dloss(base, b, tgt, interior) / eps)
+
+
+def _grad(st, rho0, p0, tgt, interior):
+    from boxzyhydro_amd.adjoint import hydro_function
+
+    f = hydro_function()
+    rho = rho0.clone().requires_grad_(True)
+    p = p0.clone().requires_grad_(True)
+    rT = f.apply(rho, p, st)
+    d = (rT - tgt) * interior
+    L = 0.5 * (d * d).sum()
+    L.backward()
+    return rho.grad.detach(), p.grad.detach(), float(L.detach())
+
+
+@pytest.mark.gpu
+def test_adjoint_gradient_matches_central_differences():
+    """64 seeded interior gradient entries (32 cells, rho and p) of a 16^3 box
+    over 8 steps with checkpoints every 3 (a partial last segment), against
+    central differences with eps = 1e-6: |fd - adjoint| <= 1e-6 max|g|.
+
+    The step is piecewise smooth (limiter argument, HLL speed bounds), so a
+    +-eps stencil can straddle a branch switch; such an entry is accepted only
+    when the two one-sided differences disagree (a kink inside the stencil)
+    and the central difference agrees at a smaller eps -- at most 4 of 64."""
+    import torch
+
+    prm, st, rho0, p0, tgt, interior = _setup()
+    g_rho, g_p, L = _grad(st, rho0, p0, tgt, interior)
+    rng = np.random.Generator(np.random.PCG64(7))
+    cells = rng.choice(np.flatnonzero(interior.cpu().numpy()), size=32, replace=False)
+    scale = max(float(g_rho.abs().max()), float(g_p.abs().max()))
+    tol = 1e-6 * scale
+    zero = torch.zeros_like(rho0)
+    kinks, worst = [], 0.0
+    for c in cells:
+        for which, g in (("rho", g_rho), ("p", g_p)):
+            e = torch.zeros_like(rho0)
+            e[c] = 1.0
+            d_rho, d_p = (e, zero) if which == "rho" else (zero, e)
+            ad = float(g[c])
+            fd, fwd, bwd = _central(st, rho0, p0, d_rho, d_p, EPS, tgt, interior)
+            err = abs(fd - ad)
+            print(f"cell {c} {which}: adjoint {ad:+.12e} fd {fd:+.12e} err/max|g| {err / scale:.2e}", flush=True)
+            if err <= tol:
+                worst = max(worst, err / scale)
+                continue
+            # a branch switch inside the stencil: the one-sided differences disagree
+            assert abs(fwd - bwd) > 10 * tol, (c, which, fd, fwd, bwd, ad)
+            refined = min(abs(_central(st, rho0, p0, d_rho, d_p, h, tgt, interior)[0] - ad) for h in (1e-7, 1e-8))
+            print(f"   kink inside +-eps (one-sided {fwd:+.6e} / {bwd:+.6e}); refined err/max|g| "
+                  f"{refined / scale:.2e}", flush=True)
+            assert refined <= tol, (c, which)
+            kinks.append((int(c), which))
+    print(f"worst smooth entry err/max|g| {worst:.2e}; entries with a kink inside +-1e-6: {kinks}", flush=True)
+    assert len(kinks) <= 4
+
+
+@pytest.mark.gpu
+def test_adjoint_matches_forward_tangent():
+    """<grad, v> against the forward's directional derivative along random
+    directions v in (rho, p), taken by central differences as eps shrinks
+    (a random direction crosses branch switches for larger eps)."""
+    import torch
+
+    prm, st, rho0, p0, tgt, interior = _setup(nsteps=10, every=4, seed=3)
+    g_rho, g_p, _ = _grad(st, rho0, p0, tgt, interior)
+    gen = torch.Generator(device="cpu").manual_seed(11)
+    for k in range(3):
+        vr = torch.randn(rho0.shape, generator=gen, dtype=torch.float64).to(rho0.device) * interior
+        vp = torch.randn(p0.shape, generator=gen, dtype=torch.float64).to(p0.device) * interior
+        ad = float((g_rho * vr).sum() + (g_p * vp).sum())
+        errs = []
+        for h in (1e-7, 1e-8, 1e-9):
+            fd = _central(st, rho0, p0, vr, vp, h, tgt, interior)[0]
+            errs.append(abs(ad - fd) / abs(fd))
+            print(f"direction {k} eps {h:.0e}: adjoint {ad:+.12e} tangent {fd:+.12e} rel {errs[-1]:.2e}", flush=True)
+        assert min(errs) <= 1e-6
+
+
+@pytest.mark.gpu
+def test_adjoint_deterministic_and_checkpoint_independent():
+    """The gradient does not depend on the checkpoint spacing and repeats bit
+    for bit."""
+    import torch
+
+    prm, st, rho0, p0, tgt, interior = _setup(nsteps=7, every=7)
+    g1 = _grad(st, rho0, p0, tgt, interior)
+    g2 = _grad(st, rho0, p0, tgt, interior)
+    assert torch.equal(g1[0], g2[0]) and torch.equal(g1[1], g2[1])
+    st.every = 2
+    g3 = _grad(st, rho0, p0, tgt, interior)
+    assert torch.equal(g1[0], g3[0]) and torch.equal(g1[1], g3[1])
+    # ghost cells carry no gradient (their values are overwritten by the fill)
+    assert float(g1[0][~interior].abs().max()) == 0.0
