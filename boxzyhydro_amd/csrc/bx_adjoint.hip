@@ -626,6 +626,7 @@ struct bx_adj {
     int32_t *slot = nullptr;  // per cell: index of its donor entry, or -1
     int ndonor = 0, nfold = 0;
     int chunk_y = 64, chunk_z = 32;
+    bool last_kept = false;  // the segment store holds the last segment's states from the forward
 };
 
 namespace bx {
@@ -850,10 +851,18 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
         if (f & FLAG_IRREGULAR) return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: " + at + " met rho<1e-4 or |u|>vlimit");
         return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: " + at + " met a floor of the update (not differentiated)");
     };
+    // the last segment's states are written straight into the segment store,
+    // so the first backward needs no recompute for it
+    const int64_t n = c->n;
+    const int s_last = every > 0 && nsteps > 0 ? ((nsteps - 1) / every) * every : nsteps;
+    double *Useg = j->seg, *Sseg = every > 0 ? j->seg + (size_t)5 * n * (every + 1) : nullptr;
+    double *Pseg = every > 0 ? Sseg + (size_t)5 * n * every : nullptr;
+    j->last_kept = false;
     for (int k = 0; k < nsteps; k++) {
         if (every > 0 && k % every == 0)
             BX_HIP(hipMemcpyAsync(j->ckpt + (size_t)(k / every) * 5 * c->n, c->cons, 5 * pb, hipMemcpyDeviceToDevice,
                                   c->stream));
+        if (k == s_last) BX_HIP(hipMemcpyAsync(Useg, c->cons, 5 * pb, hipMemcpyDeviceToDevice, c->stream));
         double dt;
         if (dts_in) dt = dts_in[k];
         else {
@@ -863,20 +872,31 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
             dtold = dt;
             dt = (dt * 0.5) * 2.0;
         }
-        if ((rc = fused_step_into(c, dt, c->cons, c->ustar, c->pstar, c->unext, !frozen))) return rc;
+        if (k >= s_last) {
+            const size_t t = (size_t)(k - s_last);
+            if ((rc = fused_step_into(c, dt, Useg + 5 * n * t, Sseg + 5 * n * t, Pseg + n * t, Useg + 5 * n * (t + 1),
+                                      !frozen)))
+                return rc;
+        } else if ((rc = fused_step_into(c, dt, c->cons, c->ustar, c->pstar, c->unext, !frozen))) {
+            return rc;
+        }
         if (!frozen) {
             if ((rc = read_flags_rate(c, &flags, &rate))) return rc;
             if (flags & (FLAG_IRREGULAR | FLAG_FLOOR)) return refuse(flags, k);
         }
-        std::swap(c->cons, c->unext);
+        if (k < s_last) std::swap(c->cons, c->unext);
         time = time + dt;
         j->dts[k] = dt;
         if (dts_out) dts_out[k] = dt;
     }
+    if (nsteps > s_last)  // the final state, from the end of the kept segment
+        BX_HIP(hipMemcpyAsync(c->cons, Useg + (size_t)5 * n * (nsteps - s_last), 5 * pb, hipMemcpyDeviceToDevice,
+                              c->stream));
     if (frozen) {
         if ((rc = read_flags_rate(c, &flags, &rate))) return rc;
         if (flags & (FLAG_IRREGULAR | FLAG_FLOOR)) return refuse(flags, nsteps - 1);
     }
+    j->last_kept = every > 0 && nsteps > s_last;
     j->nsteps = nsteps;
     c->aux_stale = true;
     c->old_stale = true;
@@ -901,9 +921,13 @@ extern "C" int bx_adjoint_backward(bx_ctx *c, const double *lam_T, double *lam_0
     double *Useg = j->seg, *Sseg = j->seg + (size_t)5 * n * (E + 1), *Pseg = Sseg + (size_t)5 * n * E;
     for (int s0 = ((j->nsteps - 1) / E) * E; s0 >= 0; s0 -= E) {
         const int len = std::min(E, j->nsteps - s0);
-        // recompute the segment from its checkpoint with the frozen dts
-        BX_HIP(hipMemcpyAsync(Useg, j->ckpt + (size_t)(s0 / E) * 5 * n, 5 * pb, hipMemcpyDeviceToDevice, c->stream));
-        for (int t = 0; t < len; t++)
+        // recompute the segment from its checkpoint with the frozen dts (the
+        // last one is still in the store after the forward)
+        const bool kept = j->last_kept && s0 + len == j->nsteps;
+        j->last_kept = false;
+        if (!kept)
+            BX_HIP(hipMemcpyAsync(Useg, j->ckpt + (size_t)(s0 / E) * 5 * n, 5 * pb, hipMemcpyDeviceToDevice, c->stream));
+        for (int t = 0; t < len && !kept; t++)
             if ((rc = fused_step_into(c, j->dts[s0 + t], Useg + (size_t)5 * n * t, Sseg + (size_t)5 * n * t,
                                       Pseg + (size_t)n * t, Useg + (size_t)5 * n * (t + 1), true)))
                 return rc;
