@@ -86,7 +86,7 @@ program boxzy_driver
 
   character(len=256) :: nmlfile, icfile, prefix, arg, mode
   integer :: maxsteps, step, n, ix, iy, iz, igrid
-  real(c_double) :: f1, dtold, tphase, yc
+  real(c_double) :: f1, dtold, tphase, yc, nextstop
   real(c_double), allocatable :: cons(:,:), u(:,:), p(:), adindx(:), dts(:)
   type(bx_params) :: prm
   integer(c_int32_t) :: st, taken
@@ -124,7 +124,11 @@ program boxzy_driver
 
   n = nx * ny * nz
   allocate(cons(5, n), u(3, n), p(n), adindx(n))
-  if (trim(icfile) == '-') then
+  if (irestart > 0) then
+     ! main.f90:83-84: restart from celldump.<irestart> (read_hydro.f90)
+     call read_hydro(irestart, step)
+     starttime = time
+  else if (trim(icfile) == '-') then
      ! init_conditions.f90:26-77: shock tube split at y=0, then the outer
      ! ghost layer gets small_rho/small_eps with p and u left at the IC
      igrid = 0
@@ -162,24 +166,30 @@ program boxzy_driver
      read(201) adindx
      close(201)
   end if
-  call bx_check(bx_upload(bxctx, cons, u, p, adindx), 'upload')
+  if (irestart <= 0) call bx_check(bx_upload(bxctx, cons, u, p, adindx), 'upload')
 
   open(unit=202, file=trim(prefix)//'.dt', access='stream', form='unformatted', status='replace')
   time = starttime
+  nextstop = starttime                            ! main.f90:102-104
+  if (irestart > 0) nextstop = nextstop + dtout
   f1 = 1d0
   dtold = 0d0
   step = 0
+  if (irestart > 0) step = irestart               ! main.f90:199
 
   if (trim(mode) == 'fused') then
-     allocate(dts(max(1, min(maxsteps, 1000000))))
-     st = 0
-     call bx_check(bx_step_c(bxctx, BX_STEP_FUSED, int(min(maxsteps, 1000000), c_int32_t), endtime, &
-          time, dtold, st, dts, taken), 'step')
-     ! reconstruct the per-step times exactly as main.f90:545 accumulates them
-     tphase = starttime
-     do step = 1, taken
-        tphase = tphase + dts(step)
-        write(202) dts(step), tphase
+     ! the whole step in one call per step (bx_step), output on main.f90:548-553's cadence
+     allocate(dts(1))
+     st = step
+     do
+        if (time >= endtime .or. st >= maxsteps .or. st >= 1000000) exit
+        call bx_check(bx_step_c(bxctx, BX_STEP_FUSED, 1_c_int32_t, endtime, time, dtold, st, dts, taken), 'step')
+        if (taken /= 1) exit
+        write(202) dts(1), time
+        if (time >= nextstop .or. time == endtime) then
+           call write_files(int(st), write_ascii)
+           nextstop = nextstop + dtout
+        end if
      end do
   else
      do  ! main.f90:211-555
@@ -215,6 +225,10 @@ program boxzy_driver
         dt = dt * 2d0
         time = time + dt
         write(202) dt, time
+        if (time >= nextstop .or. time == endtime) then  ! main.f90:548-553
+           call write_files(step, write_ascii)
+           nextstop = nextstop + dtout
+        end if
      end do
   end if
   close(202)

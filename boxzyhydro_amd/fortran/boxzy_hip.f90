@@ -77,6 +77,20 @@ module boxzy_hip
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function
+     integer(c_int) function bx_write_celldump_c(ctx, path, time, step, ascii) bind(C, name='bx_write_celldump')
+       import :: c_int, c_int32_t, c_ptr, c_double, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+       real(c_double), value :: time
+       integer(c_int32_t), value :: step, ascii
+     end function
+     integer(c_int) function bx_read_celldump_c(ctx, path, time, step) bind(C, name='bx_read_celldump')
+       import :: c_int, c_int32_t, c_ptr, c_double, c_char
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+       real(c_double), intent(out) :: time
+       integer(c_int32_t), intent(out) :: step
+     end function
      integer(c_int) function bx_flux_c(ctx, avg, dt) bind(C, name='bx_flux')
        import :: c_int, c_int32_t, c_ptr, c_double
        type(c_ptr), value :: ctx
@@ -162,6 +176,28 @@ contains
 
   subroutine copy_old()          ! main.f90:327-333
     call bx_check(bx_copy_old_c(bxctx), 'copy_old')
+  end subroutine
+
+  ! write_files.f90:5-71 (hydro part): state(), then celldump.<step> (binary,
+  ! or formatted when ascii); time is the module variable
+  subroutine write_files(step, ascii)
+    integer, intent(in) :: step
+    logical, intent(in) :: ascii
+    character(len=17) :: name
+    write(name, '(A,I8.8)') 'celldump.', step
+    call bx_check(bx_write_celldump_c(bxctx, trim(name)//c_null_char, time, int(step, c_int32_t), &
+         merge(1_c_int32_t, 0_c_int32_t, ascii)), 'write_files')
+  end subroutine
+
+  ! read_hydro.f90:4-183: restart from celldump.<irestart>; sets time and returns the file's step
+  subroutine read_hydro(irestart, step)
+    integer, intent(in) :: irestart
+    integer, intent(out) :: step
+    character(len=17) :: name
+    integer(c_int32_t) :: s
+    write(name, '(A,I8.8)') 'celldump.', irestart
+    call bx_check(bx_read_celldump_c(bxctx, trim(name)//c_null_char, time, s), 'read_hydro')
+    step = s
   end subroutine
 
 end module boxzy_hip

@@ -26,6 +26,7 @@ Host arrays are the reference's Fortran layout: ``cons`` (ngrid, 5) etc.
 from __future__ import annotations
 
 import ctypes
+import os
 from typing import Dict, Optional, Tuple
 
 import numpy as np
@@ -173,10 +174,45 @@ class HydroGrid:
         nv.check(nv.lib().bx_profile(self._h, enable, ms, ctypes.byref(steps)), "bx_profile")
         return list(ms), steps.value
 
+    def write_celldump(self, path: str, step: int, ascii: bool = False) -> None:
+        """write_files(step): state(), then the celldump file `path`."""
+        nv.check(nv.lib().bx_write_celldump(self._h, os.fsencode(path), self.time, step, int(ascii)),
+                 "bx_write_celldump")
+
+    def read_celldump(self, path: str) -> int:
+        """read_hydro(): restart from the celldump `path`; returns its step
+        and sets the driver time (starttime) from it."""
+        t = ctypes.c_double()
+        s = ctypes.c_int32()
+        nv.check(nv.lib().bx_read_celldump(self._h, os.fsencode(path), ctypes.byref(t), ctypes.byref(s)),
+                 "bx_read_celldump")
+        self.time = t.value
+        return s.value
+
     def fallback_count(self) -> int:
         c = ctypes.c_int64()
         nv.check(nv.lib().bx_fallback_count(self._h, ctypes.byref(c)), "bx_fallback_count")
         return c.value
+
+
+def celldump_encode(prm: HydroParams, path: str, time: float, step: int, cons, u, p, ascii: bool = False) -> None:
+    """Host-only celldump writer on reference-layout arrays already updated by state()."""
+    c = [np.ascontiguousarray(a, np.float64) for a in (cons, u, p)]
+    nv.check(nv.lib().bx_celldump_encode(ctypes.byref(make_params(prm)), os.fsencode(path), time, step, int(ascii),
+                                         *[nv.dptr(a) for a in c]), "bx_celldump_encode")
+
+
+def celldump_decode(prm: HydroParams, path: str):
+    """Host-only celldump reader: (time, step, arrays as read_hydro holds them
+    before its final state() call)."""
+    n = prm.ngrid
+    out = {"cons": np.empty((n, 5)), "u": np.empty((n, 3)), "p": np.empty(n), "adindx": np.empty(n)}
+    t = ctypes.c_double()
+    s = ctypes.c_int32()
+    nv.check(nv.lib().bx_celldump_decode(ctypes.byref(make_params(prm)), os.fsencode(path), ctypes.byref(t),
+                                         ctypes.byref(s), nv.dptr(out["cons"]), nv.dptr(out["u"]), nv.dptr(out["p"]),
+                                         nv.dptr(out["adindx"])), "bx_celldump_decode")
+    return t.value, s.value, out
 
 
 def step_sequence(grid, nsteps: int, endtime: float) -> Tuple[list, list]:

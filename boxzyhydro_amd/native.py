@@ -86,6 +86,11 @@ _SIGS = {
     "bx_adjoint_backward": [_VP, _VP, _VP],
     "bx_adjoint_tune": [_VP, _I32, _I32],
     "bx_cons_device": [_VP, _VP, _I32],
+    "bx_write_celldump": [_VP, ctypes.c_char_p, ctypes.c_double, _I32, _I32],
+    "bx_read_celldump": [_VP, ctypes.c_char_p, _DP, ctypes.POINTER(_I32)],
+    "bx_celldump_encode": [ctypes.POINTER(BxParams), ctypes.c_char_p, ctypes.c_double, _I32, _I32, _DP, _DP, _DP],
+    "bx_celldump_decode": [ctypes.POINTER(BxParams), ctypes.c_char_p, _DP, ctypes.POINTER(_I32), _DP, _DP, _DP,
+                           _DP],
 }
 
 EXPORTED = tuple(_SIGS) + ("bx_last_error",)

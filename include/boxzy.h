@@ -207,6 +207,29 @@ int bx_adjoint_tune(bx_ctx *ctx, int32_t y_chunk, int32_t z_chunk);
  * recomputed by the next step), 0 reads it.  Synchronises. */
 int bx_cons_device(bx_ctx *ctx, double *planes, int32_t to_ctx);
 
+/* ------------------------------------------------------------------------
+ * celldump files (write_files.f90:5-71 hydro part; read_hydro.f90:4-183
+ * without DOUBLEGRID).  Binary: Fortran unformatted sequential records with
+ * 4-byte record lengths -- 1234; time, step; the 8 units values; per cell
+ * x, y, z, rho, p, u1, u2, u3, phi, E, muc (float64) and boundary (int32).
+ * ascii != 0: the formatted form of write_files.f90:44-49.
+ *
+ * bx_write_celldump replaces write_files(step): it runs state() on the
+ * context (as write_files.f90:31-33 does) and writes the file `path`.
+ * bx_read_celldump replaces read_hydro(): it reads `path`, rebuilds the
+ * momenta from u, resets adindx and the frame cells (read_hydro.f90:150-178),
+ * uploads, and runs state(); *time and *step receive the file's values.
+ * The encode/decode pair is the host-only part on host arrays (Fortran
+ * layout): encode writes arrays that state() has already updated; decode
+ * returns the arrays read_hydro holds before its final state() call.
+ */
+int bx_write_celldump(bx_ctx *ctx, const char *path, double time, int32_t step, int32_t ascii);
+int bx_read_celldump(bx_ctx *ctx, const char *path, double *time, int32_t *step);
+int bx_celldump_encode(const bx_params *prm, const char *path, double time, int32_t step, int32_t ascii,
+                       const double *cons, const double *u, const double *p);
+int bx_celldump_decode(const bx_params *prm, const char *path, double *time, int32_t *step, double *cons,
+                       double *u, double *p, double *adindx);
+
 /* Synchronise the context's stream. */
 int bx_sync(bx_ctx *ctx);
 

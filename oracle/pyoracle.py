@@ -181,11 +181,14 @@ def have_reference() -> bool:
 
 
 def run_reference(prm, ic: Dict[str, np.ndarray], maxsteps: int, mode: int = 0,
-                  threads: int = 4, diag: int = 0, workdir: Optional[str] = None) -> Dict[str, np.ndarray]:
+                  threads: int = 4, diag: int = 0, workdir: Optional[str] = None,
+                  celldump: int = 0) -> Dict[str, np.ndarray]:
     """Run the reference's own subroutines (oracle/_ref/ref_harness) on ``ic``.
 
     Returns dt/time per step and the final cons/u/p; with mode=1 also the
-    snapshots after every sub-call of step 1."""
+    snapshots after every sub-call of step 1; with celldump=1 (binary) or 2
+    (ASCII) also the bytes of the celldump that write_files writes after the
+    run (``res["celldump"]``)."""
     if not have_reference():
         raise RuntimeError("reference driver not built (make -C oracle ref)")
     from boxzyhydro_amd.ics import write_ic_file
@@ -196,8 +199,8 @@ def run_reference(prm, ic: Dict[str, np.ndarray], maxsteps: int, mode: int = 0,
             f.write(prm.to_namelist())
         write_ic_file(os.path.join(td, "ic.bin"), ic)
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-        subprocess.run([REF_HARNESS, "run.nml", "ic.bin", "out", str(maxsteps), str(mode), str(diag)],
-                       cwd=td, env=env, check=True, stdout=subprocess.DEVNULL)
+        subprocess.run([REF_HARNESS, "run.nml", "ic.bin", "out", str(maxsteps), str(mode), str(diag),
+                        str(celldump)], cwd=td, env=env, check=True, stdout=subprocess.DEVNULL)
         dt = np.fromfile(os.path.join(td, "out.dt"), np.float64).reshape(-1, 2)
         fin = np.fromfile(os.path.join(td, "out.final"), np.float64)
         tim = open(os.path.join(td, "out.timing")).read().split()
@@ -220,4 +223,28 @@ def run_reference(prm, ic: Dict[str, np.ndarray], maxsteps: int, mode: int = 0,
                               "p": arr[1 + 8 * n:1 + 9 * n].copy(),
                               "cons_old": arr[1 + 9 * n:1 + 14 * n].reshape(n, 5).copy()})
             res["snaps"] = snaps
+        if celldump:
+            with open(os.path.join(td, f"celldump.{len(dt):08d}"), "rb") as f:
+                res["celldump"] = f.read()
         return res
+
+
+def reference_restart(prm, celldump: bytes, step: int, threads: int = 1,
+                      workdir: Optional[str] = None) -> Dict[str, np.ndarray]:
+    """The reference's read_hydro() (through oracle/_ref/ref_harness, mode 2)
+    on the binary celldump ``celldump`` saved as celldump.<step>: the
+    cons/u/p it leaves after its state() call."""
+    if not have_reference():
+        raise RuntimeError("reference driver not built (make -C oracle ref)")
+    n = prm.ngrid
+    with tempfile.TemporaryDirectory(dir=workdir) as td:
+        with open(os.path.join(td, "run.nml"), "w") as f:
+            f.write(prm.to_namelist())
+        with open(os.path.join(td, f"celldump.{step:08d}"), "wb") as f:
+            f.write(celldump)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        subprocess.run([REF_HARNESS, "run.nml", "none", "out", str(step), "2"], cwd=td, env=env, check=True,
+                       stdout=subprocess.DEVNULL)
+        fin = np.fromfile(os.path.join(td, "out.final"), np.float64)
+        return {"cons": fin[:5 * n].reshape(n, 5).copy(), "u": fin[5 * n:8 * n].reshape(n, 3).copy(),
+                "p": fin[8 * n:9 * n].copy()}

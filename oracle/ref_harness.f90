@@ -8,7 +8,7 @@
 ! It is never shipped and never called by the product path; its outputs are
 ! golden vectors for the C restatement (boxzy_oracle.c) and the HIP kernels.
 !
-! Usage:  ref_harness <namelist> <ic_file> <out_prefix> <max_steps> <mode> [diag]
+! Usage:  ref_harness <namelist> <ic_file> <out_prefix> <max_steps> <mode> [diag] [dump]
 !   ic_file   stream-binary float64: cons(5,ngrid) u(3,ngrid) p(ngrid) adindx(ngrid)
 !             (Fortran order, exactly the reference's grid_commons arrays)
 !   mode 0    run steps; write <prefix>.dt (dt, time per step),
@@ -16,7 +16,11 @@
 !             <prefix>.timing (wall seconds of the step loop, steps, threads)
 !   mode 1    run steps; additionally write <prefix>.snap with the state after
 !             every sub-call of the first step (tagged records)
+!   mode 2    restart: read_hydro() reads celldump.<max_steps> (irestart =
+!             max_steps) instead of ic_file; write <prefix>.final and stop
 !   diag 1    also run the per-step conservation diagnostics (main.f90:222-223)
+!   dump 1|2  after the run, write_files(step) with write_ascii .false.|.true.
+!             (celldump.<step> in the working directory)
 !
 ! avisc (avisc.f90:4-26) is not driven here: for outer-layer ghosts it reads
 ! grid(0) and u(:,0) (ineigh(2:6)=0 from init_grid.f90:178), outside the
@@ -37,7 +41,7 @@ program ref_harness
 
  character(len=256)::nmlfile,icfile,prefix,arg
  character(len=80)::nml80
- integer::maxsteps,mode,diag,step,igrid,idim,ios
+ integer::maxsteps,mode,diag,dump,step,igrid,idim,ios
  real(pre)::f1,dtold,tphase
  real(pre)::etot_loc0,mass_loc0,f_loc0
  real(8)::wt0,wt1
@@ -52,6 +56,10 @@ program ref_harness
  if(command_argument_count()>=6)then
    call get_command_argument(6,arg); read(arg,*)diag
  endif
+ dump=0
+ if(command_argument_count()>=7)then
+   call get_command_argument(7,arg); read(arg,*)dump
+ endif
 
  nml80=nmlfile(1:80)
  call read_params(nml80)
@@ -61,6 +69,17 @@ program ref_harness
 
 ! initial conditions from file, then the bookkeeping that init_conditions.f90
 ! performs for every set-up: phi=0, gforce from phi (-> -0.0), cons_old=cons
+ if(mode==2)then
+   irestart=maxsteps
+   phi=zero
+   call read_hydro()
+   open(unit=205,file=trim(prefix)//'.final',access='stream',form='unformatted',status='replace')
+   write(205)cons
+   write(205)u
+   write(205)p
+   close(205)
+   stop
+ endif
  open(unit=201,file=trim(icfile),access='stream',form='unformatted',status='old',iostat=ios)
  if(ios/=0)stop "cannot open ic file"
  read(201)cons
@@ -190,6 +209,11 @@ program ref_harness
  write(205)u
  write(205)p
  close(205)
+
+ if(dump>0)then
+   write_ascii=(dump==2)
+   call write_files(step)
+ endif
 
 contains
 

@@ -1,0 +1,127 @@
+"""celldump writer and restart reader against the reference's own
+write_files() and read_hydro() (golden vectors in tests/golden/celldump.json,
+made by tests/golden/make_celldump_golden.py): byte-identical files, binary
+and ASCII, and a bit-identical restart state."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from cases import case_inputs, sha
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CD = json.load(open(os.path.join(HERE, "golden", "celldump.json")))
+
+
+def _digest(path):
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def _oracle_final(golden, name):
+    import pyoracle as po
+
+    rec = CD[name]
+    prm, ic = case_inputs(golden[name])
+    og = po.OracleGrid(prm, ic)
+    po.set_threads(1)
+    dts, times = og.run(rec["steps"])
+    og.state()  # write_files.f90:31-33
+    assert times[-1] == rec["time"]
+    return prm, og, times[-1]
+
+
+@pytest.mark.parametrize("name", sorted(CD))
+@pytest.mark.parametrize("ascii", [False, True])
+def test_encoder_matches_reference_file(golden, name, ascii, tmp_path):
+    """Host encoder on the oracle's final state (bit-identical to the
+    reference's) writes the reference's file byte for byte."""
+    from boxzyhydro_amd.hydro import celldump_encode
+
+    prm, og, t = _oracle_final(golden, name)
+    path = str(tmp_path / "celldump")
+    celldump_encode(prm, path, t, CD[name]["steps"], og.cons, og.u, og.p, ascii=ascii)
+    want = CD[name]["ascii" if ascii else "binary"]
+    assert os.path.getsize(path) == want["bytes"]
+    assert _digest(path) == want["sha256"]
+
+
+@pytest.mark.parametrize("name", sorted(CD))
+def test_decoder_matches_reference_restart(golden, name, tmp_path):
+    """read_hydro(): decode, then the oracle's state(), equals the reference's
+    restart state bit for bit."""
+    import pyoracle as po
+
+    from boxzyhydro_amd.hydro import celldump_decode, celldump_encode
+
+    prm, og, t = _oracle_final(golden, name)
+    path = str(tmp_path / "celldump")
+    celldump_encode(prm, path, t, CD[name]["steps"], og.cons, og.u, og.p)
+    tt, step, arr = celldump_decode(prm, path)
+    assert tt == t and step == CD[name]["steps"]
+    rs = po.OracleGrid(prm, arr)
+    rs.state()  # read_hydro.f90:180
+    for k in ("cons", "u", "p"):
+        assert sha(getattr(rs, k)) == CD[name]["restart_sha256"][k], k
+
+
+def test_decoder_rebuilds_momenta_and_frame(golden, tmp_path):
+    """Closed forms of read_hydro.f90:150-178: m = u rho, adindx = gammafix,
+    frame cells small_rho / 0 / small_eps; rho, E, u, p as written."""
+    from boxzyhydro_amd.hydro import celldump_decode, celldump_encode
+    from boxzyhydro_amd.ics import boundary_flags
+
+    prm, og, t = _oracle_final(golden, "stress8")
+    path = str(tmp_path / "celldump")
+    celldump_encode(prm, path, t, 3, og.cons, og.u, og.p)
+    _, _, arr = celldump_decode(prm, path)
+    frame = boundary_flags(prm).reshape(-1) == 1
+    inner = ~frame
+    assert np.array_equal(arr["cons"][inner, 0], og.cons[inner, 0])
+    assert np.array_equal(arr["cons"][inner, 4], og.cons[inner, 4])
+    assert np.array_equal(arr["u"], og.u) and np.array_equal(arr["p"], og.p)
+    for k in range(3):
+        assert np.array_equal(arr["cons"][inner, 1 + k], og.u[inner, k] * og.cons[inner, 0])
+    assert np.all(arr["cons"][frame] == [prm.small_rho, 0.0, 0.0, 0.0, prm.small_eps])
+    assert np.all(arr["adindx"] == prm.gammafix)
+
+
+def test_decoder_rejects_truncated_file(golden, tmp_path):
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import celldump_decode, celldump_encode
+
+    prm, og, t = _oracle_final(golden, "flatz_walls")
+    path = str(tmp_path / "celldump")
+    celldump_encode(prm, path, t, 4, og.cons, og.u, og.p)
+    raw = open(path, "rb").read()
+    with open(path, "wb") as f:
+        f.write(raw[:-50])
+    with pytest.raises(nv.BoxzyError, match="short file"):
+        celldump_decode(prm, path)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CD))
+def test_gpu_write_and_restart_match_reference(golden, name, tmp_path):
+    """write_files/read_hydro through the C-ABI on a GPU run of the case."""
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+
+    rec = CD[name]
+    prm, ic = case_inputs(golden[name])
+    print(f"celldump case {name}", flush=True)
+    g = HydroGrid(prm, ic, device=0)
+    g.run(rec["steps"], mode=nv.BX_STEP_FUSED)
+    assert g.time == rec["time"]
+    for ascii in (False, True):
+        path = str(tmp_path / f"celldump{int(ascii)}")
+        g.write_celldump(path, rec["steps"], ascii=ascii)
+        assert _digest(path) == rec["ascii" if ascii else "binary"]["sha256"]
+    r = HydroGrid(prm, None, device=0)
+    step = r.read_celldump(str(tmp_path / "celldump0"))
+    assert step == rec["steps"] and r.time == rec["time"]
+    out = r.download()
+    for k in ("cons", "u", "p"):
+        assert sha(out[k]) == rec["restart_sha256"][k], k

@@ -14,15 +14,19 @@ from conftest import ROOT
 DRIVER = os.path.join(ROOT, "boxzyhydro_amd", "fortran", "boxzy_driver")
 
 
-def run_driver(tmp_path, prm, ic, mode, steps=1000000):
+def run_driver(tmp_path, prm, ic, mode, steps=1000000, restart=0):
+    """Run the driver in tmp_path (its celldump files land there)."""
     nml = tmp_path / "run.nml"
-    nml.write_text(prm.to_namelist())
+    text = prm.to_namelist()
+    if restart:
+        text = text.replace("irestart=0", f"irestart={restart}")
+    nml.write_text(text)
     icf = "-"
     if ic is not None:
         icf = str(tmp_path / "ic.bin")
         ics.write_ic_file(icf, ic)
     out = str(tmp_path / "out")
-    subprocess.run([DRIVER, str(nml), icf, out, str(steps), mode], check=True, timeout=300)
+    subprocess.run([DRIVER, str(nml), icf, out, str(steps), mode], check=True, timeout=300, cwd=str(tmp_path))
     dt = np.fromfile(out + ".dt", np.float64).reshape(-1, 2)
     fin = np.fromfile(out + ".final", np.float64)
     n = prm.ngrid
@@ -58,3 +62,31 @@ def test_driver_builtin_setup_matches_reference_readme_run(golden, tmp_path):
     assert list(dts) == rec["dt"]
     for k in ("cons", "u", "p"):
         assert sha(out[k]) == rec["final_sha256"][k], k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["calls", "fused"])
+def test_driver_writes_reference_celldumps_and_restarts(golden, tmp_path, mode):
+    """write_files on main.f90:548-553's cadence (dtout > endtime: after step 1
+    and at endtime) gives the reference's celldump files byte for byte, and a
+    restart from the last one (read_hydro) gives the reference's restart state."""
+    import hashlib
+    import json
+
+    cd = json.load(open(os.path.join(ROOT, "tests", "golden", "celldump.json")))
+    rec = golden["sod_t02"]
+    prm, ic = case_inputs(rec)
+    print(f"fortran driver celldumps {mode}", flush=True)
+    run_driver(tmp_path, prm, ic, mode, rec["steps"])
+    files = sorted(f.name for f in tmp_path.glob("celldump.*"))
+    assert files == ["celldump.00000001", f"celldump.{rec['steps']:08d}"]
+
+    def digest(name):
+        return hashlib.sha256((tmp_path / name).read_bytes()).hexdigest()
+
+    assert digest("celldump.00000001") == cd["sod_step1"]["binary"]["sha256"]
+    assert digest(files[-1]) == cd["sod_t02"]["binary"]["sha256"]
+    # restart: no further step (maxsteps = irestart), the state read_hydro leaves
+    _, out = run_driver(tmp_path, prm, None, mode, rec["steps"], restart=rec["steps"])
+    for k in ("cons", "u", "p"):
+        assert sha(out[k]) == cd["sod_t02"]["restart_sha256"][k], k
