@@ -310,6 +310,7 @@ int bx_destroy(bx_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     adjoint_free(c);
+    eos_free(c);
     void *dev[] = {c->cons, c->cons_old, c->u, c->p, c->adindx, c->qq, c->states, c->ghost_dst, c->ghost_src,
                    c->ghost_sign, c->ghost_scratch_src, c->ghost_scratch, c->ustar, c->pstar, c->unext, c->dsc,
                    c->cons32, c->ustar32, c->unext32, c->pstar32, c->eflux};

@@ -21,23 +21,14 @@
 #include <vector>
 
 #include "bx_internal.h"
+#include "bx_units.h"
 
 namespace {
 
-// units.f90:10-17 with parameters.f90:7-11 (gcgs, rgasCGS, pi)
 void get_units(double s[8])
 {
-    const double gcgs = 6.67e-8, rgas_cgs = 8.254e7, pi = 3.1415926535897931;
-    const double length = 1.496e13;
-    const double time = (3600.0 * 365.25 * 24.0) / (2.0 * pi);
-    const double density = 1.0 / (gcgs * (time * time));
-    const double mass = density * ((length * length) * length);
-    const double vel = length / time;
-    const double eps = ((vel * vel) * mass) / ((length * length) * length);
-    const double kelvin = 1.0;
-    const double rgas = rgas_cgs / ((vel * vel) / kelvin);
-    // declaration order of type units: length, time, mass, density, rgas, eps, kelvin, vel
-    const double v[8] = {length, time, mass, density, rgas, eps, kelvin, vel};
+    const bx::Units u = bx::get_units();  // declaration order of type units
+    const double v[8] = {u.length, u.time, u.mass, u.density, u.rgas, u.eps, u.kelvin, u.vel};
     for (int i = 0; i < 8; i++) s[i] = v[i];
 }
 
@@ -92,7 +83,7 @@ std::string fort_e(double v, int w, int d, int e)
 }  // namespace
 
 extern "C" int bx_celldump_encode(const bx_params *prm, const char *path, double time, int32_t step, int32_t ascii,
-                                  const double *cons, const double *u, const double *p)
+                                  const double *cons, const double *u, const double *p, const double *muc_array)
 {
     if (!prm || !path || !cons || !u || !p) return bx::set_error(BX_ERR_ARG, "bx_celldump_encode: null argument");
     const int nx = prm->nx, ny = prm->ny, nz = prm->nz;
@@ -127,7 +118,7 @@ extern "C" int bx_celldump_encode(const bx_params *prm, const char *path, double
             for (int ix = 1; ix <= nx && ok; ix++, c++) {
                 const double v[11] = {centre(prm->dx, ix, nx), centre(prm->dy, iy, ny), centre(prm->dz, iz, nz),
                                       cons[c * 5], p[c], u[c * 3], u[c * 3 + 1], u[c * 3 + 2], phi, cons[c * 5 + 4],
-                                      muc};
+                                      muc_array ? muc_array[c] : muc};
                 const int32_t bd = boundary_of(ix, iy, iz, nx, ny, nz);
                 if (ascii) {
                     std::string line;
@@ -212,9 +203,14 @@ extern "C" int bx_write_celldump(bx_ctx *c, const char *path, double time, int32
     int rc;
     if ((rc = bx_state(c))) return rc;
     const size_t n = (size_t)c->n;
-    std::vector<double> cons(5 * n), u(3 * n), p(n);
+    std::vector<double> cons(5 * n), u(3 * n), p(n), muc;
     if ((rc = bx_download(c, cons.data(), u.data(), p.data(), nullptr))) return rc;
-    return bx_celldump_encode(&c->prm, path, time, step, ascii, cons.data(), u.data(), p.data());
+    if (c->eos) {  // per-cell muc_array of the tabulated EOS
+        muc.resize(n);
+        if ((rc = bx_download_muc(c, muc.data()))) return rc;
+    }
+    return bx_celldump_encode(&c->prm, path, time, step, ascii, cons.data(), u.data(), p.data(),
+                              c->eos ? muc.data() : nullptr);
 }
 
 extern "C" int bx_read_celldump(bx_ctx *c, const char *path, double *time, int32_t *step)

@@ -48,7 +48,7 @@ __device__ __forceinline__ void block_max_to(double v, unsigned long long *dst)
 }
 
 // ------------------------------------------------------------------ state
-// state.f90:109-139, H2STAT=-1 branch
+// state.f90:30-57, H2STAT=-1 branch
 __global__ __launch_bounds__(kThreads) void k_state(int64_t n, double *__restrict__ cons,
                                                    const double *__restrict__ u, double *__restrict__ p,
                                                    double *__restrict__ ad, double gam, double seps,
@@ -375,6 +375,7 @@ __global__ __launch_bounds__(kThreads) void k_avisc(int64_t n, int nx, int ny, i
 // ------------------------------------------------------------- launchers
 int launch_state(bx_ctx *c)
 {
+    if (c->eos) return launch_state_tab(c);  // H2STAT >= 0 (bx_eos.hip)
     hipLaunchKernelGGL(k_state, dim3(blocks_for(c->n, kThreads)), dim3(kThreads), 0, c->stream, c->n, c->cons,
                        c->u, c->p, c->adindx, c->prm.gammafix, c->prm.small_eps, c->muc, c->rgas);
     BX_HIP(hipGetLastError());

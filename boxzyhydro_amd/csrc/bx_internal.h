@@ -93,6 +93,7 @@ struct bx_ctx {
     bool state_in_32;  // the current state lives in cons32 (mixed form)
 
     struct bx_adj *adj;  // adjoint store (bx_adjoint.hip), or null
+    struct bx_eos *eos;  // tabulated EOS (bx_eos.hip), or null for a fixed gamma
 };
 
 namespace bx {
@@ -126,6 +127,8 @@ int fused_step(bx_ctx *c, double dt);
 // first, the flags too when reset_flags (else they accumulate over steps)
 int fused_step_into(bx_ctx *c, double dt, double *u0, double *us, double *ps, double *un, bool reset_flags);
 void adjoint_free(bx_ctx *c);
+void eos_free(bx_ctx *c);
+int launch_state_tab(bx_ctx *c);
 int fused_prepare(bx_ctx *c);
 int narrow_cons(bx_ctx *c);
 int widen_cons(bx_ctx *c);

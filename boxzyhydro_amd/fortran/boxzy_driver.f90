@@ -89,6 +89,7 @@ program boxzy_driver
   real(c_double) :: f1, dtold, tphase, yc, nextstop
   real(c_double), allocatable :: cons(:,:), u(:,:), p(:), adindx(:), dts(:)
   type(bx_params) :: prm
+  type(bx_eos_params) :: eosprm
   integer(c_int32_t) :: st, taken
   logical :: frame
 
@@ -106,7 +107,6 @@ program boxzy_driver
   if (command_argument_count() >= 5) call get_command_argument(5, mode)
 
   call read_params(trim(nmlfile))
-  if (H2STAT /= -1) stop 'boxzy_driver: only H2STAT=-1 (fixed gamma) is on the GPU path'
   if (fluxangmom) stop 'boxzy_driver: fluxangmom is not on the GPU path'
 
   ! init_grid.f90 counterpart: the context owns the device arrays and tables
@@ -121,6 +121,14 @@ program boxzy_driver
        merge(1, 0, no_outflow_zl), merge(1, 0, no_outflow_zr)]
   prm%precision = BX_PREC_FP64
   call bx_check(bx_create(prm, 0_c_int32_t, bxctx), 'create')
+  if (H2STAT >= 0) then
+     ! main.f90:70-71: initialize_eos + calc_eos_table (tables kept on the device)
+     call bx_eos_defaults(eosprm)
+     eosprm%H2STAT = H2STAT; eosprm%NEOS_RHO = NEOS_RHO
+     eosprm%dTk_eos = dTk_eos; eosprm%tk_bgrnd = tk_bgrnd; eosprm%brot = brot; eosprm%vib = vib
+     eosprm%ac = ac; eosprm%bc = bc
+     call bx_check(bx_eos_init(bxctx, eosprm), 'eos_init')
+  end if
 
   n = nx * ny * nz
   allocate(cons(5, n), u(3, n), p(n), adindx(n))

@@ -28,6 +28,13 @@ module boxzy_hip
      integer(c_int32_t) :: precision
   end type bx_params
 
+  ! &eos_input keys and the table bounds of parameters.f90:44-58
+  type, bind(C) :: bx_eos_params
+     integer(c_int32_t) :: H2STAT, NEOS_RHO
+     real(c_double) :: dTk_eos, tk_bgrnd, brot, vib, ac, bc
+     real(c_double) :: diss, ionize, rho_eos_low, rho_eos_high, tk_eos_cutoff
+  end type bx_eos_params
+
   interface
      function bx_last_error() bind(C, name='bx_last_error') result(p)
        import :: c_ptr
@@ -83,6 +90,20 @@ module boxzy_hip
        character(kind=c_char), intent(in) :: path(*)
        real(c_double), value :: time
        integer(c_int32_t), value :: step, ascii
+     end function
+     subroutine bx_eos_defaults(e) bind(C, name='bx_eos_defaults')
+       import :: bx_eos_params
+       type(bx_eos_params), intent(out) :: e
+     end subroutine
+     integer(c_int) function bx_eos_init(ctx, e) bind(C, name='bx_eos_init')
+       import :: c_int, c_ptr, bx_eos_params
+       type(c_ptr), value :: ctx
+       type(bx_eos_params), intent(in) :: e
+     end function
+     integer(c_int) function bx_download_muc(ctx, muc_array) bind(C, name='bx_download_muc')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: muc_array(*)
      end function
      integer(c_int) function bx_read_celldump_c(ctx, path, time, step) bind(C, name='bx_read_celldump')
        import :: c_int, c_int32_t, c_ptr, c_double, c_char

@@ -51,6 +51,37 @@ def make_params(prm: HydroParams, precision: int = nv.BX_PREC_FP64) -> nv.BxPara
     return P
 
 
+def make_eos_params(prm: HydroParams) -> nv.BxEosParams:
+    """bx_eos_params of ``prm``: the &eos_input keys, the rest at the
+    reference's parameters.f90 values."""
+    E = nv.BxEosParams()
+    nv.lib().bx_eos_defaults(ctypes.byref(E))
+    E.H2STAT, E.NEOS_RHO = prm.H2STAT, prm.NEOS_RHO
+    E.dTk_eos, E.tk_bgrnd, E.brot, E.vib, E.ac, E.bc = (prm.dTk_eos, prm.tk_bgrnd, prm.brot, prm.vib,
+                                                         prm.ac, prm.bc)
+    return E
+
+
+def eos_tables(prm: HydroParams) -> Dict[str, np.ndarray]:
+    """Host-only: the tables calc_eos_table (eos.f90:127-520) leaves for
+    ``prm``; 2-D tables have shape (NEOS_RHO, NEOS_T)."""
+    E = make_eos_params(prm)
+    nt = ctypes.c_int32()
+    nv.check(nv.lib().bx_eos_size(ctypes.byref(E), ctypes.byref(nt)), "bx_eos_size")
+    nt, nr = nt.value, prm.NEOS_RHO
+    out = {"scalars": np.empty(2), "rho_table": np.empty(nr), "deng_eos_array": np.empty(nr),
+           "tk_table": np.empty(nt)}
+    names2 = ("eng_table", "gamma_table", "muc_table", "eng_table2", "tk_table2", "muc_table2", "gamma_table2")
+    for k in names2:
+        out[k] = np.empty((nr, nt))
+    order = ("scalars", "rho_table", "deng_eos_array", "tk_table") + names2
+    nv.check(nv.lib().bx_eos_tables(ctypes.byref(make_params(prm)), ctypes.byref(E),
+                                    *[nv.dptr(out[k]) for k in order]), "bx_eos_tables")
+    out["log_rho_eos_low"], out["drho_eos"] = float(out["scalars"][0]), float(out["scalars"][1])
+    out["NEOS_T"], out["NEOS_RHO"] = nt, nr
+    return out
+
+
 class HydroGrid:
     """Device-resident state of one grid plus the reference's step scalars."""
 
@@ -69,6 +100,8 @@ class HydroGrid:
         self.time = prm.starttime
         self.dtold = 0.0
         self.step_count = 0
+        if prm.H2STAT >= 0:  # main.f90:70-71: initialize_eos + calc_eos_table
+            nv.check(nv.lib().bx_eos_init(self._h, ctypes.byref(make_eos_params(prm))), "bx_eos_init")
         if ic is not None:
             self.upload(ic)
 
@@ -101,6 +134,12 @@ class HydroGrid:
         out = {"cons": np.empty((n, 5)), "u": np.empty((n, 3)), "p": np.empty(n), "adindx": np.empty(n)}
         nv.check(nv.lib().bx_download(self._h, nv.dptr(out["cons"]), nv.dptr(out["u"]),
                                       nv.dptr(out["p"]), nv.dptr(out["adindx"])), "bx_download")
+        return out
+
+    def download_muc(self) -> np.ndarray:
+        """muc_array (grid_commons.f90:12) as the last state() left it."""
+        out = np.empty(self.ngrid)
+        nv.check(nv.lib().bx_download_muc(self._h, nv.dptr(out)), "bx_download_muc")
         return out
 
     def download_cons_old(self) -> np.ndarray:
@@ -195,9 +234,11 @@ class HydroGrid:
         return c.value
 
 
-def celldump_encode(prm: HydroParams, path: str, time: float, step: int, cons, u, p, ascii: bool = False) -> None:
-    """Host-only celldump writer on reference-layout arrays already updated by state()."""
-    c = [np.ascontiguousarray(a, np.float64) for a in (cons, u, p)]
+def celldump_encode(prm: HydroParams, path: str, time: float, step: int, cons, u, p, ascii: bool = False,
+                    muc=None) -> None:
+    """Host-only celldump writer on reference-layout arrays already updated by
+    state(); ``muc`` is muc_array (None: the fixed-gamma muc everywhere)."""
+    c = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (cons, u, p, muc)]
     nv.check(nv.lib().bx_celldump_encode(ctypes.byref(make_params(prm)), os.fsencode(path), time, step, int(ascii),
                                          *[nv.dptr(a) for a in c]), "bx_celldump_encode")
 

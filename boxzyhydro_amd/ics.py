@@ -122,6 +122,50 @@ def box_ic(prm: HydroParams, seed: int = 0, sigma_cells: float = 4.0,
     return _assemble(prm, rho, vel[0], vel[1], vel[2], p)
 
 
+def h2_cloud_ic(prm: HydroParams, seed: int = 0, sigma_cells: float = 2.0, rho0: float = 0.1,
+                rho_sigma: float = 1.0, t_lo: float = 20.0, t_hi: float = 4000.0,
+                mach_rms: float = 0.5) -> Dict[str, np.ndarray]:
+    """Seeded molecular-cloud box for the tabulated H2 EOS (H2STAT >= 0), in
+    code units (units.f90): ``rho = rho0 exp(rho_sigma g1)`` and a temperature
+    log-uniform-ish in [t_lo, t_hi] from ``g2`` (g1, g2 smoothed unit Gaussian
+    fields); ``p = rho R T / muc`` (eos.f90:144 muc) and ``E = p/(gammafix-1)
+    + ekin``, so the first state() call finds T through the tables.  rms
+    Mach ``mach_rms`` against sqrt(1.4 <p>/<rho>)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shp = prm.shape
+    g1 = _smooth_unit(rng.standard_normal(shp), sigma_cells)
+    g2 = _smooth_unit(rng.standard_normal(shp), sigma_cells)
+    rho = rho0 * np.exp(rho_sigma * g1)
+    lt = 0.5 * (np.log(t_lo) + np.log(t_hi)) + 0.25 * (np.log(t_hi) - np.log(t_lo)) * g2
+    tk = np.exp(np.clip(lt, np.log(t_lo), np.log(t_hi)))
+    p = rho * prm.rgas * tk / prm.muc
+    c = np.sqrt(1.4 * p.mean() / rho.mean())
+    vel = rng.standard_normal((3,) + shp)
+    vel *= (mach_rms * c) / np.sqrt(np.mean(np.sum(vel * vel, axis=0)))
+    if prm.nz == 1:
+        vel[2] = 0.0
+    return _assemble(prm, rho, vel[0], vel[1], vel[2], p)
+
+
+def h2_disk_ic(prm: HydroParams, seed: int = 0, omega: float = 0.05, **kw) -> Dict[str, np.ndarray]:
+    """A single-layer (nz=1) rotating disk for the tabulated EOS's disk branch
+    (state.f90:61-69, which needs a nonzero kinetic energy): the cloud of
+    ``h2_cloud_ic`` plus solid rotation u = omega (-y, x, 0)."""
+    if prm.nz != 1:
+        raise ValueError("h2_disk_ic: nz must be 1")
+    ic = h2_cloud_ic(prm, seed=seed, **kw)
+    x, y, _ = cell_centres(prm)
+    X = np.broadcast_to(x[None, None, :], prm.shape).reshape(-1)
+    Y = np.broadcast_to(y[None, :, None], prm.shape).reshape(-1)
+    rho = ic["cons"][:, 0]
+    p = ic["p"]
+    vx = ic["u"][:, 0] - omega * Y
+    vy = ic["u"][:, 1] + omega * X
+    zero = np.zeros_like(vx)
+    return _assemble(prm, rho.reshape(prm.shape), vx.reshape(prm.shape), vy.reshape(prm.shape),
+                     zero.reshape(prm.shape), p.reshape(prm.shape))
+
+
 def _riemann_draws(n_problems: int, seed: int, lmin: int, lmax: int):
     rng = np.random.Generator(np.random.PCG64(seed))
     lens = np.rint(np.exp(rng.uniform(np.log(lmin), np.log(lmax), n_problems))).astype(int)

@@ -38,6 +38,20 @@ class BxParams(ctypes.Structure):
     ]
 
 
+class BxEosParams(ctypes.Structure):
+    """Mirror of ``bx_eos_params`` (include/boxzy.h)."""
+
+    _fields_ = [
+        ("H2STAT", ctypes.c_int32), ("NEOS_RHO", ctypes.c_int32),
+        ("dTk_eos", ctypes.c_double), ("tk_bgrnd", ctypes.c_double),
+        ("brot", ctypes.c_double), ("vib", ctypes.c_double),
+        ("ac", ctypes.c_double), ("bc", ctypes.c_double),
+        ("diss", ctypes.c_double), ("ionize", ctypes.c_double),
+        ("rho_eos_low", ctypes.c_double), ("rho_eos_high", ctypes.c_double),
+        ("tk_eos_cutoff", ctypes.c_double),
+    ]
+
+
 class BoxzyError(RuntimeError):
     pass
 
@@ -88,12 +102,17 @@ _SIGS = {
     "bx_cons_device": [_VP, _VP, _I32],
     "bx_write_celldump": [_VP, ctypes.c_char_p, ctypes.c_double, _I32, _I32],
     "bx_read_celldump": [_VP, ctypes.c_char_p, _DP, ctypes.POINTER(_I32)],
-    "bx_celldump_encode": [ctypes.POINTER(BxParams), ctypes.c_char_p, ctypes.c_double, _I32, _I32, _DP, _DP, _DP],
+    "bx_celldump_encode": [ctypes.POINTER(BxParams), ctypes.c_char_p, ctypes.c_double, _I32, _I32, _DP, _DP, _DP,
+                           _DP],
     "bx_celldump_decode": [ctypes.POINTER(BxParams), ctypes.c_char_p, _DP, ctypes.POINTER(_I32), _DP, _DP, _DP,
                            _DP],
+    "bx_eos_size": [ctypes.POINTER(BxEosParams), ctypes.POINTER(_I32)],
+    "bx_eos_tables": [ctypes.POINTER(BxParams), ctypes.POINTER(BxEosParams)] + [_DP] * 11,
+    "bx_eos_init": [_VP, ctypes.POINTER(BxEosParams)],
+    "bx_download_muc": [_VP, _DP],
 }
 
-EXPORTED = tuple(_SIGS) + ("bx_last_error",)
+EXPORTED = tuple(_SIGS) + ("bx_last_error", "bx_eos_defaults")
 
 
 def lib() -> ctypes.CDLL:
@@ -113,6 +132,8 @@ def lib() -> ctypes.CDLL:
             fn.restype = ctypes.c_int
         L.bx_last_error.argtypes = []
         L.bx_last_error.restype = ctypes.c_char_p
+        L.bx_eos_defaults.argtypes = [ctypes.POINTER(BxEosParams)]
+        L.bx_eos_defaults.restype = None
         _lib = L
     return _lib
 

@@ -42,13 +42,22 @@ class HydroParams:
     avmagx: float = 0.0
     avmagy: float = 0.0
     avmagz: float = 0.0
-    # &eos_input (input.f90:15-16); only H2STAT=-1 (fixed gamma) is on the path
+    # &eos_input (input.f90:14-15).  H2STAT=-1: fixed gamma; 0: H2 with a fixed
+    # ortho:para ratio ac:bc; 1: H2 in ortho/para equilibrium (eos.f90:179-290).
+    # The tabulated forms (0, 1) also use the table keys below.
     gammafix: float = 1.66666666667
     xabun: float = 0.73
     yabun: float = 0.27
     zabun: float = 0.0
     mu_z: float = 16.78
     H2STAT: int = -1
+    NEOS_RHO: int = 1000
+    dTk_eos: float = 5.0
+    tk_bgrnd: float = 1e-6
+    brot: float = 85.4
+    vib: float = 5987.0
+    ac: float = 1.0
+    bc: float = 3.0
 
     @property
     def ngrid(self) -> int:
@@ -115,11 +124,19 @@ class HydroParams:
             f"H2STAT={self.H2STAT}", f"gammafix={f(self.gammafix)}",
             f"xabun={f(self.xabun)}", f"yabun={f(self.yabun)}", f"zabun={f(self.zabun)}",
             f"mu_z={f(self.mu_z)}",
-            # tiny unused EOS tables: H2STAT=-1 only needs muc (eos.f90:144)
-            "NEOS_RHO=4", "dTk_eos=1d4",
+            *self._table_keys(f),
             "/", "",
             "&particle_input", "npart=0", "/", "",
         ])
+
+    def _table_keys(self, f):
+        if self.H2STAT < 0:
+            # tiny unused tables: H2STAT=-1 only needs muc (eos.f90:144), and
+            # main.f90:70-71 builds the tables whatever H2STAT is
+            return ["NEOS_RHO=4", "dTk_eos=1d4"]
+        return [f"NEOS_RHO={self.NEOS_RHO}", f"dTk_eos={f(self.dTk_eos)}",
+                f"tk_bgrnd={f(self.tk_bgrnd)}", f"brot={f(self.brot)}", f"vib={f(self.vib)}",
+                f"ac={f(self.ac)}", f"bc={f(self.bc)}"]
 
     @classmethod
     def from_namelist(cls, text: str) -> "HydroParams":

@@ -75,7 +75,7 @@ int bx_download(bx_ctx *ctx, double *cons, double *u, double *p, double *adindx)
 int bx_download_cons_old(bx_ctx *ctx, double *cons_old);
 
 /* ---- one entry point per reference subroutine on the path ---- */
-int bx_state(bx_ctx *ctx);                        /* state.f90:6-89 (H2STAT=-1)         */
+int bx_state(bx_ctx *ctx);                        /* state.f90:6-89 (H2STAT -1, 0, 1)   */
 int bx_velocity(bx_ctx *ctx, double *maxv);       /* velocity.f90:4-64; maxv may be NULL */
 int bx_courant(bx_ctx *ctx, double *dt);          /* courant.f90:4-39: dt = cfl/max rate */
 int bx_source(bx_ctx *ctx, double tphase, int32_t siter, double dt); /* source.f90:5-205 */
@@ -220,15 +220,53 @@ int bx_cons_device(bx_ctx *ctx, double *planes, int32_t to_ctx);
  * momenta from u, resets adindx and the frame cells (read_hydro.f90:150-178),
  * uploads, and runs state(); *time and *step receive the file's values.
  * The encode/decode pair is the host-only part on host arrays (Fortran
- * layout): encode writes arrays that state() has already updated; decode
- * returns the arrays read_hydro holds before its final state() call.
+ * layout): encode writes arrays that state() has already updated (muc_array
+ * NULL: the fixed-gamma muc of eos.f90:144 in every cell); decode returns the
+ * arrays read_hydro holds before its final state() call.
  */
 int bx_write_celldump(bx_ctx *ctx, const char *path, double time, int32_t step, int32_t ascii);
 int bx_read_celldump(bx_ctx *ctx, const char *path, double *time, int32_t *step);
 int bx_celldump_encode(const bx_params *prm, const char *path, double time, int32_t step, int32_t ascii,
-                       const double *cons, const double *u, const double *p);
+                       const double *cons, const double *u, const double *p, const double *muc_array);
 int bx_celldump_decode(const bx_params *prm, const char *path, double *time, int32_t *step, double *cons,
                        double *u, double *p, double *adindx);
+
+/* ------------------------------------------------------------------------
+ * Tabulated H2 equation of state (H2STAT = 0: fixed ortho:para ratio ac:bc;
+ * 1: ortho/para equilibrium), eos.f90.  The keys of &eos_input
+ * (input.f90:14-15) and the fixed table bounds of parameters.f90:44-58;
+ * xabun, yabun, zabun, mu_z and gammafix come from bx_params.
+ */
+typedef struct bx_eos_params {
+    int32_t H2STAT;    /* -1 fixed gamma (no tables), 0 or 1 */
+    int32_t NEOS_RHO;  /* density rows of the tables */
+    double dTk_eos, tk_bgrnd, brot, vib, ac, bc;                  /* &eos_input */
+    double diss, ionize, rho_eos_low, rho_eos_high, tk_eos_cutoff; /* parameters.f90 */
+} bx_eos_params;
+
+/* The reference's defaults (parameters.f90:44-62).  Host-only. */
+void bx_eos_defaults(bx_eos_params *e);
+/* NEOS_T = int((tk_eos_cutoff - tk_bgrnd)/dTk_eos) + 1 (eos.f90:33).  Host-only. */
+int bx_eos_size(const bx_eos_params *e, int32_t *neos_t);
+/* Host-only: the tables initialize_eos() + calc_eos_table() (eos.f90:30-51,
+ * 127-520) leave, bit-identical, in the reference's units after the call.
+ * scalars[2] = log_rho_eos_low, drho_eos; rho_table and deng_eos_array hold
+ * NEOS_RHO values, tk_table NEOS_T; the 2-D tables NEOS_T x NEOS_RHO in
+ * Fortran order.  Any output may be NULL. */
+int bx_eos_tables(const bx_params *prm, const bx_eos_params *e, double *scalars, double *rho_table,
+                  double *deng_eos_array, double *tk_table, double *eng_table, double *gamma_table,
+                  double *muc_table, double *eng_table2, double *tk_table2, double *muc_table2,
+                  double *gamma_table2);
+/* Replaces initialize_eos + calc_eos_table (main.f90:70-71): builds the tables
+ * on the host, keeps them on the device, and makes bx_state() take the
+ * tabulated branch (state.f90:58-79, get_gamma2 eos.f90:653-699), which also
+ * sets muc_array.  H2STAT=-1 restores the fixed-gamma branch.  FP64 contexts,
+ * before the first fused step; with tables, bx_step runs the exact sequence in
+ * either mode (the fused kernels assume a fixed gamma) and the adjoint is
+ * unavailable. */
+int bx_eos_init(bx_ctx *ctx, const bx_eos_params *e);
+/* muc_array(ngrid) (grid_commons.f90:12) as the last state() left it. */
+int bx_download_muc(bx_ctx *ctx, double *muc_array);
 
 /* Synchronise the context's stream. */
 int bx_sync(bx_ctx *ctx);

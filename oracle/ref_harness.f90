@@ -18,6 +18,13 @@
 !             every sub-call of the first step (tagged records)
 !   mode 2    restart: read_hydro() reads celldump.<max_steps> (irestart =
 !             max_steps) instead of ic_file; write <prefix>.final and stop
+!   mode 3    write the EOS tables calc_eos_table() leaves to <prefix>.eos and
+!             stop: NEOS_T, NEOS_RHO (int32); log_rho_eos_low, drho_eos (f64);
+!             rho_table, deng_eos_array (NEOS_RHO); tk_table (NEOS_T);
+!             eng_table, gamma_table, muc_table, eng_table2, tk_table2,
+!             muc_table2, gamma_table2 (NEOS_T x NEOS_RHO, Fortran order)
+!   mode 4    one state() call on ic_file; write <prefix>.final and stop
+!   every run also writes <prefix>.eos_state: adindx, muc_array (ngrid each)
 !   diag 1    also run the per-step conservation diagnostics (main.f90:222-223)
 !   dump 1|2  after the run, write_files(step) with write_ascii .false.|.true.
 !             (celldump.<step> in the working directory)
@@ -69,15 +76,21 @@ program ref_harness
 
 ! initial conditions from file, then the bookkeeping that init_conditions.f90
 ! performs for every set-up: phi=0, gforce from phi (-> -0.0), cons_old=cons
+ if(mode==3)then
+   open(unit=207,file=trim(prefix)//'.eos',access='stream',form='unformatted',status='replace')
+   write(207)NEOS_T,NEOS_RHO
+   write(207)log_rho_eos_low,drho_eos
+   write(207)rho_table,deng_eos_array,tk_table
+   write(207)eng_table,gamma_table,muc_table
+   write(207)eng_table2,tk_table2,muc_table2,gamma_table2
+   close(207)
+   stop
+ endif
  if(mode==2)then
    irestart=maxsteps
    phi=zero
    call read_hydro()
-   open(unit=205,file=trim(prefix)//'.final',access='stream',form='unformatted',status='replace')
-   write(205)cons
-   write(205)u
-   write(205)p
-   close(205)
+   call write_final()
    stop
  endif
  open(unit=201,file=trim(icfile),access='stream',form='unformatted',status='old',iostat=ios)
@@ -92,6 +105,14 @@ program ref_harness
  call calc_gforce()
 !$OMP END PARALLEL
  cons_old=cons
+
+ if(mode==4)then
+!$OMP PARALLEL DEFAULT(SHARED)
+   call state()
+!$OMP END PARALLEL
+   call write_final()
+   stop
+ endif
 
  if(mode==1)then
    open(unit=203,file=trim(prefix)//'.snap',access='stream',form='unformatted',status='replace')
@@ -204,11 +225,7 @@ program ref_harness
  write(206,*)wt1-wt0,step,omp_get_max_threads()
  close(206)
 
- open(unit=205,file=trim(prefix)//'.final',access='stream',form='unformatted',status='replace')
- write(205)cons
- write(205)u
- write(205)p
- close(205)
+ call write_final()
 
  if(dump>0)then
    write_ascii=(dump==2)
@@ -216,6 +233,18 @@ program ref_harness
  endif
 
 contains
+
+ subroutine write_final()
+  open(unit=205,file=trim(prefix)//'.final',access='stream',form='unformatted',status='replace')
+  write(205)cons
+  write(205)u
+  write(205)p
+  close(205)
+  open(unit=208,file=trim(prefix)//'.eos_state',access='stream',form='unformatted',status='replace')
+  write(208)adindx
+  write(208)muc_array
+  close(208)
+ end subroutine
 
  subroutine snap(tag)
   integer,intent(in)::tag

@@ -46,6 +46,32 @@ def stress_ic(prm: HydroParams, seed: int):
     return ic
 
 
+def eos_cells_ic(prm: HydroParams, seed: int):
+    """Independent random cells for one call of the tabulated state()
+    (state.f90:58-79): densities inside the tables, below them, above them and
+    under the low-density threshold 1e-4; specific energies from below the
+    first table row to above the last; some cells with E below the kinetic
+    energy (the small_eps floor); random adindx (read by the nz=1 branch)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = prm.ngrid
+    kind = rng.integers(0, 10, n)
+    lr = np.where(kind < 7, rng.uniform(-4, 4, n),
+                  np.where(kind == 7, rng.uniform(-9.5, -4, n), rng.uniform(4, 10, n)))
+    rho = 10.0 ** lr
+    e = 10.0 ** rng.uniform(-6, 1, n)
+    u = rng.standard_normal((n, 3)) * np.sqrt(e)[:, None]
+    ekin = rho * 0.5 * (u[:, 0] ** 2 + u[:, 1] ** 2 + u[:, 2] ** 2)
+    E = rho * e + ekin
+    floor = kind == 9
+    E[floor] = 0.5 * ekin[floor]
+    cons = np.empty((n, 5))
+    cons[:, 0] = rho
+    cons[:, 1:4] = rho[:, None] * u
+    cons[:, 4] = E
+    return {"cons": cons, "u": u, "p": 10.0 ** rng.uniform(-6, 0, n),
+            "adindx": rng.uniform(1.1, 1.67, n)}
+
+
 def stress_tubes():
     """Shock tubes that take the exact branches of the step: a vacuum-forming
     rarefaction (state.f90:52-56 low-density branch, density and energy
@@ -108,6 +134,12 @@ def case_inputs(rec: dict):
         ic = ics.box_ic(prm, seed=gen["seed"], sigma_cells=gen["sigma_cells"])
     elif kind == "stress":
         ic = stress_ic(prm, gen["seed"])
+    elif kind == "eos_cells":
+        ic = eos_cells_ic(prm, gen["seed"])
+    elif kind == "h2_cloud":
+        ic = ics.h2_cloud_ic(prm, seed=gen["seed"], **gen.get("kw", {}))
+    elif kind == "h2_disk":
+        ic = ics.h2_disk_ic(prm, seed=gen["seed"], **gen.get("kw", {}))
     elif kind == "riemann_batch":
         batch = ics.riemann_batch(gen["n_problems"], seed=gen["seed"], lmin=gen["lmin"], lmax=gen["lmax"])
         prm2, ic = batch[gen["index"]]
