@@ -12,6 +12,7 @@ Prints one JSON line: the FP64 cell-updates/s (value), the HBM roofline of
 the dominant kernel (the corrector's x/z kernel) timed with HIP events on the
 library's stream, the mixed-precision form on the same box, the batched
 shock tubes (512 problems, one persistent launch replayed from a hipGraph),
+the tabulated H2 EOS (state() kernel rate, step cost against a fixed gamma),
 the adjoint/forward cost (128^3 box, 200 steps, checkpoints every 20) and a
 CPU baseline: the reference's own Fortran subroutines (oracle/_ref) timed on
 the host on a bounded sample, or the C restatement when that build is absent.
@@ -211,6 +212,66 @@ def adjoint_cost(n: int, nsteps: int, every: int):
             "finite": bool(np.isfinite(float(g_rho.abs().max())))}
 
 
+B_STATE_TAB = 64.0  # tabulated state(): read rho, E, u (5 x 8 B), write p, adindx, muc_array (3 x 8 B)
+
+
+def eos_cost(n: int, steps: int, reps: int):
+    """The tabulated H2 EOS (H2STAT=0, the reference's default 10000 x 1000
+    tables) on an n^3 molecular-cloud box: host table build, the state()
+    kernel's HBM rate, and the exact-sequence step with tables against the
+    same step with a fixed gamma."""
+    import dataclasses
+
+    from boxzyhydro_amd import ics
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+    from boxzyhydro_amd.params import box_params
+
+    prm = dataclasses.replace(box_params(n), H2STAT=0, gammafix=1.4, tk_bgrnd=10.0)
+    ic = ics.h2_cloud_ic(prm, seed=0)
+    t0 = time.perf_counter()
+    g = HydroGrid(prm, ic, device=0)  # bx_eos_init builds the tables on the host and uploads them
+    t_init = time.perf_counter() - t0
+    for _ in range(3):
+        g.state()
+    g.sync()
+    best = float("inf")
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g.state()
+        g.sync()
+        best = min(best, (time.perf_counter() - t0) / reps)
+    ncell = prm.ngrid
+    interior = prm.interior_cells
+    g.run(1, mode=nv.BX_STEP_EXACT)
+    g.sync()
+    t0 = time.perf_counter()
+    g.run(steps, mode=nv.BX_STEP_EXACT)
+    g.sync()
+    t_tab = (time.perf_counter() - t0) / steps
+    g.close()
+    prm_fix = dataclasses.replace(prm, H2STAT=-1)
+    g = HydroGrid(prm_fix, ic, device=0)
+    g.run(1, mode=nv.BX_STEP_EXACT)
+    g.sync()
+    t0 = time.perf_counter()
+    g.run(steps, mode=nv.BX_STEP_EXACT)
+    g.sync()
+    t_fix = (time.perf_counter() - t0) / steps
+    g.close()
+    bw = B_STATE_TAB * ncell / best
+    return {"workload": f"H2 cloud box {n}^3 interior, H2STAT=0, tables 10000 T x 1000 rho (parameters.f90 "
+                        f"sizes, tk_bgrnd 10 K), exact call sequence",
+            "table_build_and_upload_s": t_init,
+            "state_kernel": {"ms": best * 1e3, "cells": ncell, "alg_bytes_per_cell": B_STATE_TAB,
+                             "achieved_GBps": bw / 1e9, "hbm_frac": bw / HBM_PEAK,
+                             "note": "wall time of back-to-back bx_state calls (one kernel each) / calls"},
+            "exact_step_ms_tabulated": t_tab * 1e3, "exact_step_ms_fixed_gamma": t_fix * 1e3,
+            "tabulated_cell_updates_per_s": interior / t_tab,
+            "tabulated_over_fixed_step_time": t_tab / t_fix}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,6 +286,8 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision line")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched shock tubes")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost")
+    ap.add_argument("--no-eos", action="store_true", help="skip the tabulated-EOS line")
+    ap.add_argument("--eos-only", action="store_true", help="run only the tabulated-EOS line")
     ap.add_argument("--adjoint-size", type=int, default=128)
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--mixed-study", action="store_true",
@@ -255,6 +318,9 @@ def main() -> int:
     from boxzyhydro_amd.hydro import HydroGrid
     from boxzyhydro_amd.params import box_params
 
+    if args.eos_only:
+        print(json.dumps({"eos": eos_cost(args.size, 5, 20)}))
+        return 0
     prm = box_params(args.size)
     log(f"generating box {args.size}^3 ...")
     ic = ics.box_ic(prm, seed=0)
@@ -320,6 +386,9 @@ def main() -> int:
     if not args.no_batch:
         log("batched shock tubes ...")
         out["batched"] = batched(512, 3)
+    if not args.no_eos:
+        log("tabulated EOS ...")
+        out["eos"] = eos_cost(args.size, 5, 20)
     if not args.no_adjoint:
         log(f"adjoint {args.adjoint_size}^3 x {args.adjoint_steps} ...")
         out["adjoint"] = adjoint_cost(args.adjoint_size, args.adjoint_steps, 20)
