@@ -513,8 +513,8 @@ int enqueue(bx_batch *b)
 
 void drop_graph(bx_batch *b)
 {
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
     b->graph_valid = false;
@@ -529,7 +529,7 @@ int capture(bx_batch *b)
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return rc;
     }
     BX_HIP(e);

@@ -512,7 +512,7 @@ int bx_flux(bx_ctx *c, int32_t avg, double dt)
     if (avg != 0 && avg != 1) return set_error(BX_ERR_ARG, "bx_flux: avg must be 0 or 1");
     int rc = ensure_aux(c);
     if (rc) return rc;
-    if (!c->states) BX_HIP(hipMalloc((void **)&c->states, sizeof(double) * 42 * (size_t)c->n));
+    if (!c->states) BX_HIP(hipMalloc((void **)&c->states, sizeof(double) * state_planes(c) * (size_t)c->n));
     return launch_flux(c, avg, dt);
 }
 
@@ -555,7 +555,7 @@ static int ghosts_again(bx_ctx *c) { return c->nscratch > 0 ? launch_ghosts(c, c
 static int exact_step(bx_ctx *c, double endtime, double *time, double *dtold, int32_t *step, double *dt_out)
 {
     int rc;
-    if (!c->states) BX_HIP(hipMalloc((void **)&c->states, sizeof(double) * 42 * (size_t)c->n));
+    if (!c->states) BX_HIP(hipMalloc((void **)&c->states, sizeof(double) * state_planes(c) * (size_t)c->n));
     *step += 1;
     if ((rc = launch_state(c))) return rc;  // :232
     double dt;

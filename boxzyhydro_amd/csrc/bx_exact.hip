@@ -129,13 +129,19 @@ __global__ __launch_bounds__(kThreads) void k_courant(int64_t n, const double *_
     block_max_to(r0, rate);
 }
 
+// centred coordinate of cell index i (1-based) on an axis of n cells (init_grid.f90:85-93)
+__device__ __forceinline__ double centre(double d, int i, int n) { return d * ((double)i - ((double)n + 1.0) / 2.0); }
+
 // ----------------------------------------------------------------- source
 // source.f90:148-182 on interior cells with a uniform gravity vector g
-// (the reference's g = -(phi(+)-phi(-))/(2d) = -0.0 for phi=0)
+// (the reference's g = -(phi(+)-phi(-))/(2d) = -0.0 for phi=0); with
+// fluxangmom the terms of source.f90:73-90 come first (the pforce of
+// source.f90:64-66 feeds nothing on the hydro path)
 __global__ __launch_bounds__(kThreads) void k_source(int64_t n, int nx, int ny, int nz,
                                                     double *__restrict__ cons,
                                                     const double *__restrict__ u, double dt,
-                                                    double g0, double g1, double g2, double seps)
+                                                    double g0, double g1, double g2, double seps,
+                                                    int ang, double dx, double dy)
 {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
@@ -145,6 +151,14 @@ __global__ __launch_bounds__(kThreads) void k_source(int64_t n, int nx, int ny, 
     const double rho = cons[c];
     double m0 = cons[n + c], m1 = cons[2 * n + c], m2 = cons[3 * n + c];
     const double ux = u[c], uy = u[n + c], uz = u[2 * n + c];
+    if (ang) {
+        const double x = centre(dx, ix, nx), y = centre(dy, iy, ny);
+        const double r = sqrt(x * x + y * y);
+        // r**4 as the reference's build evaluates it: a left-to-right product
+        const double w = x * uy - y * ux, r4 = ((r * r) * r) * r;
+        m1 = m1 + rho * (w * w) * y / r4 * dt;
+        m0 = m0 + rho * (w * w) * x / r4 * dt;
+    }
     m1 = m1 + (rho * g1) * dt;
     m0 = m0 + (rho * g0) * dt;
     m2 = m2 + (g2 * rho) * dt;
@@ -204,6 +218,10 @@ __global__ __launch_bounds__(kThreads) void k_ghosts(int64_t n, int64_t ng, cons
 // flux.f90:47-116: slopes (calculate_slopes, 253-295) and interface states
 // (calculate_states, 312-482) of every cell; ghosts get zero slopes.
 // Output: 42 planes, index (dir*14 + k), k = 0..6 minus (rho,u,v,w,p,eps,gamma), 7..13 plus.
+// ANG (flux_ang.f90:301-482): 54 planes, index (dir*18 + k), k = 0..8 minus
+// (rho,u,v,w,p,eps,gamma,rmom,amom), 9..17 plus; rmom and amom are the
+// transformed cons(2), cons(3) and get no transverse term.
+template <bool ANG>
 __global__ __launch_bounds__(kThreads) void k_flux_states(int64_t n, int nx, int ny, int nz, bool flatz,
                                                          const double *__restrict__ cons,
                                                          const double *__restrict__ u,
@@ -227,8 +245,9 @@ __global__ __launch_bounds__(kThreads) void k_flux_states(int64_t n, int nx, int
     const double eps = vmax(E - 0.5 * rho * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), seps);
     const double sound = sqrt(pc * gc / rho);
 
+    constexpr int NS = ANG ? 9 : 7;
     for (int d = 0; d < 3; d++) {
-        double su[3] = {0.0, 0.0, 0.0}, sp = 0.0, sd = 0.0, se = 0.0, sgm = 0.0;
+        double su[3] = {0.0, 0.0, 0.0}, sp = 0.0, sd = 0.0, se = 0.0, sgm = 0.0, srm = 0.0, sam = 0.0;
         if (!ghost) {
             const int64_t cm = c - stride[d], cp = c + stride[d];
             for (int j = 0; j < 3; j++) su[j] = slope(u[j * n + cm], v[j], u[j * n + cp]);
@@ -247,19 +266,31 @@ __global__ __launch_bounds__(kThreads) void k_flux_states(int64_t n, int nx, int
             sd = slope(cons[cm], rho, cons[cp]);
             se = slope(em, eps, ep);
             sgm = slope(ad[cm], gc, ad[cp]);
+            if (ANG) {  // flux_ang.f90:319-320
+                srm = slope(cons[n + cm], cons[n + c], cons[n + cp]);
+                sam = slope(cons[2 * n + cm], cons[2 * n + c], cons[2 * n + cp]);
+            }
         }
         const double lp = vmax(v[d] + sound, 0.0) * dt / ds[d];
         const double lm = vmin(v[d] - sound, 0.0) * dt / ds[d];
         const double zc = 0.0 * (dt * 0.5 / ds[d]);  // transverse terms: flux.f90:391-479
-        double mi[7], pl[7];
+        double mi[NS], pl[NS];
         muscl(rho, lm, lp, sd, mi[0], pl[0]);
         for (int j = 0; j < 3; j++) muscl(v[j], lm, lp, su[j], mi[1 + j], pl[1 + j]);
         muscl(pc, lm, lp, sp, mi[4], pl[4]);
         muscl(eps, lm, lp, se, mi[5], pl[5]);
         muscl(gc, lm, lp, sgm, mi[6], pl[6]);
         for (int k = 0; k < 7; k++) {
-            st[(int64_t)(d * 14 + k) * n + c] = mi[k] + zc;
-            st[(int64_t)(d * 14 + 7 + k) * n + c] = pl[k] + zc;
+            st[(int64_t)(d * 2 * NS + k) * n + c] = mi[k] + zc;
+            st[(int64_t)(d * 2 * NS + NS + k) * n + c] = pl[k] + zc;
+        }
+        if constexpr (ANG) {  // flux_ang.f90:377-380
+            muscl(cons[n + c], lm, lp, srm, mi[7], pl[7]);
+            muscl(cons[2 * n + c], lm, lp, sam, mi[8], pl[8]);
+            for (int k = 7; k < 9; k++) {
+                st[(int64_t)(d * 2 * NS + k) * n + c] = mi[k];
+                st[(int64_t)(d * 2 * NS + NS + k) * n + c] = pl[k];
+            }
         }
     }
 }
@@ -287,15 +318,103 @@ __device__ __forceinline__ void dir_fluxes(const double *__restrict__ st, int64_
     face_flux<double, dir>(load_side(st, n, dir, 1, c), load_side(st, n, dir, 0, c + s), f1);
 }
 
+// flux_ang.f90:523-618 (get_fluxes_ang): the momentum slots carry the
+// reconstructed r- and angular momenta, and the pressure force is projected
+// with the centre (x, y, r) of the cell whose faces these are.
+template <int dir>
+__device__ __forceinline__ void face_flux_ang(const double *__restrict__ st, int64_t n, int64_t cl, int64_t cr,
+                                              double x, double y, double r, double F[5])
+{
+    const double *bl = st + (int64_t)(dir * 18 + 9) * n + cl;  // plus state of the left cell
+    const double *br = st + (int64_t)(dir * 18) * n + cr;      // minus state of the right cell
+    const double dl = bl[0], vxl = bl[n], vyl = bl[2 * n], vzl = bl[3 * n], pl = bl[4 * n], el = bl[5 * n],
+                 gl = bl[6 * n];
+    const double dr = br[0], vxr = br[n], vyr = br[2 * n], vzr = br[3 * n], pr = br[4 * n], er = br[5 * n],
+                 gr = br[6 * n];
+    double sl[5], sr[5], fl[5], fr[5];
+    sl[0] = dl;
+    sl[1] = bl[7 * n];
+    sl[2] = bl[8 * n];
+    sl[3] = dl * vzl;
+    sl[4] = el + 0.5 * dl * (vxl * vxl + vyl * vyl + vzl * vzl);
+    sr[0] = dr;
+    sr[1] = br[7 * n];
+    sr[2] = br[8 * n];
+    sr[3] = dr * vzr;
+    sr[4] = er + 0.5 * dr * (vxr * vxr + vyr * vyr + vzr * vzr);
+    double vl, vr, pl1 = 0.0, pl2 = 0.0, pl3 = 0.0, pr1 = 0.0, pr2 = 0.0, pr3 = 0.0;
+    if (dir == 0) {
+        vl = vxl;
+        vr = vxr;
+        pl1 = pl * x / r;
+        pr1 = pr * x / r;
+        pl2 = -(pl * y);
+        pr2 = -(pr * y);
+    } else if (dir == 1) {
+        vl = vyl;
+        vr = vyr;
+        pl1 = pl * y / r;
+        pr1 = pr * y / r;
+        pl2 = pl * x;
+        pr2 = pr * x;
+    } else {
+        vl = vzl;
+        vr = vzr;
+        pl3 = pl;
+        pr3 = pr;
+    }
+    fl[0] = sl[0] * vl;
+    fl[1] = pl1 + sl[1] * vl;
+    fl[2] = pl2 + sl[2] * vl;
+    fl[3] = pl3 + sl[3] * vl;
+    fl[4] = (sl[4] + pl) * vl;
+    fr[0] = sr[0] * vr;
+    fr[1] = pr1 + sr[1] * vr;
+    fr[2] = pr2 + sr[2] * vr;
+    fr[3] = pr3 + sr[3] * vr;
+    fr[4] = (sr[4] + pr) * vr;
+    // flux_hll, flux.f90:583-589
+    const double cR = sqrt(gr * pr / dr), cL = sqrt(gl * pl / dl);
+    const double ap = vmax(vmax(0.0, vr + cR), vl + cL);
+    const double am = vmax(vmax(0.0, -(vr - cR)), -(vl - cL));
+    for (int i = 0; i < 5; i++) F[i] = (ap * fl[i] + am * fr[i] - ap * am * (sr[i] - sl[i])) / (ap + am);
+}
+
+// flux_ang.f90:40-61 (and 245-268 inverted): cons(2:3), and cons_old(2:3)
+// when avg>0, to r- and angular momentum about the z axis through y=-yoffset
+__global__ __launch_bounds__(kThreads) void k_ang_to(int64_t n, int nx, int ny, double *__restrict__ cons,
+                                                    double *__restrict__ cons_old, int avg, double dx, double dy,
+                                                    double yoffset)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int ix = (int)(c % nx) + 1, iy = (int)((c / nx) % ny) + 1;
+    const double x = centre(dx, ix, nx), y = centre(dy, iy, ny) + yoffset;
+    const double r = sqrt(x * x + y * y);
+    for (int k = 0; k < 1 + avg; k++) {
+        double *q = k == 0 ? cons : cons_old;
+        const double m2 = q[n + c], m3 = q[2 * n + c];
+        const double angmom = x * m3 - y * m2;
+        const double rmom = (m2 * x + m3 * y) / r;
+        q[n + c] = rmom;
+        q[2 * n + c] = angmom;
+    }
+}
+
 // ------------------------------------------------- flux: faces and update
 // flux.f90:121-230: six HLL faces and fluxtmp on interior cells (fluxtmp=0 on
 // ghosts, flux.f90:51), then the stage update and floors on every cell.
+// ANG: flux_ang.f90:62-281 -- faces from face_flux_ang, the update in the
+// transformed momenta, back to linear momenta (and cons_old when avg>0),
+// then the floors.
+template <bool ANG>
 __global__ __launch_bounds__(kThreads) void k_flux_faces(int64_t n, int nx, int ny, int nz, bool flatz,
                                                         const double *__restrict__ st,
                                                         double *__restrict__ cons,
-                                                        const double *__restrict__ cons_old, int avg,
+                                                        double *__restrict__ cons_old, int avg,
                                                         double dt, double ayz, double axz, double axy,
-                                                        double vol, double srho, double seps)
+                                                        double vol, double srho, double seps, double dx,
+                                                        double dy, double yoffset)
 {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
@@ -304,9 +423,21 @@ __global__ __launch_bounds__(kThreads) void k_flux_faces(int64_t n, int nx, int 
     double ftmp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (boundary_of(ix, iy, iz, nx, ny, nz) == 0) {
         double fx0[5], fx1[5], fy0[5], fy1[5], fz0[5], fz1[5];
-        dir_fluxes<0>(st, n, c, 1, fx0, fx1);
-        dir_fluxes<1>(st, n, c, nx, fy0, fy1);
-        dir_fluxes<2>(st, n, c, flatz ? 0 : (int64_t)nx * ny, fz0, fz1);
+        const int64_t sz = flatz ? 0 : (int64_t)nx * ny;
+        if constexpr (ANG) {
+            const double x = centre(dx, ix, nx), y = centre(dy, iy, ny);  // get_fluxes_ang: no yoffset
+            const double r = sqrt(x * x + y * y);
+            face_flux_ang<0>(st, n, c - 1, c, x, y, r, fx0);
+            face_flux_ang<0>(st, n, c, c + 1, x, y, r, fx1);
+            face_flux_ang<1>(st, n, c - nx, c, x, y, r, fy0);
+            face_flux_ang<1>(st, n, c, c + nx, x, y, r, fy1);
+            face_flux_ang<2>(st, n, c - sz, c, x, y, r, fz0);
+            face_flux_ang<2>(st, n, c, c + sz, x, y, r, fz1);
+        } else {
+            dir_fluxes<0>(st, n, c, 1, fx0, fx1);
+            dir_fluxes<1>(st, n, c, nx, fy0, fy1);
+            dir_fluxes<2>(st, n, c, sz, fz0, fz1);
+        }
         for (int i = 0; i < 5; i++)
             ftmp[i] = -((ayz * (fx1[i] - fx0[i]) + axz * (fy1[i] - fy0[i]) + axy * (fz1[i] - fz0[i])) * dt / vol);
     }
@@ -315,6 +446,19 @@ __global__ __launch_bounds__(kThreads) void k_flux_faces(int64_t n, int nx, int 
         const double ft = ftmp[i];
         const double cu = cons[(int64_t)i * n + c];
         U[i] = avg == 0 ? cu + ft : 0.5 * (cons_old[(int64_t)i * n + c] + cu + ft);
+    }
+    if constexpr (ANG) {  // flux_ang.f90:249-265
+        const double x = centre(dx, ix, nx), y = centre(dy, iy, ny) + yoffset;
+        const double r = sqrt(x * x + y * y);
+        double angmom = U[2], rmom = U[1];
+        U[1] = (rmom * x - angmom * y / r) / r;
+        U[2] = (rmom * y + angmom * x / r) / r;
+        if (avg > 0) {
+            angmom = cons_old[2 * n + c];
+            rmom = cons_old[n + c];
+            cons_old[n + c] = (rmom * x - angmom * y / r) / r;
+            cons_old[2 * n + c] = (rmom * y + angmom * x / r) / r;
+        }
     }
     U[0] = vmax(U[0], srho);
     if (U[4] < 0.0) U[4] = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0] + seps;
@@ -419,7 +563,8 @@ int launch_ghosts(bx_ctx *c, double *planes5)
 int launch_source(bx_ctx *c, double dt)
 {
     hipLaunchKernelGGL(k_source, dim3(blocks_for(c->n, kThreads)), dim3(kThreads), 0, c->stream, c->n, c->nx,
-                       c->ny, c->nz, c->cons, c->u, dt, c->grav[0], c->grav[1], c->grav[2], c->prm.small_eps);
+                       c->ny, c->nz, c->cons, c->u, dt, c->grav[0], c->grav[1], c->grav[2], c->prm.small_eps,
+                       c->prm.fluxangmom, c->prm.dx, c->prm.dy);
     BX_HIP(hipGetLastError());
     return launch_ghosts(c, c->cons);
 }
@@ -427,14 +572,22 @@ int launch_source(bx_ctx *c, double dt)
 int launch_flux(bx_ctx *c, int avg, double dt)
 {
     const double dx = c->prm.dx, dy = c->prm.dy, dz = c->prm.dz;
-    hipLaunchKernelGGL(k_flux_states, dim3(blocks_for(c->n, kThreads)), dim3(kThreads), 0, c->stream, c->n,
-                       c->nx, c->ny, c->nz, c->flatz, c->cons, c->u, c->p, c->adindx, c->states, dt, dx, dy, dz,
-                       c->prm.small_eps);
+    const bool ang = c->prm.fluxangmom != 0;
+    const dim3 grid(blocks_for(c->n, kThreads));
+    if (ang) {
+        hipLaunchKernelGGL(k_ang_to, grid, dim3(kThreads), 0, c->stream, c->n, c->nx, c->ny, c->cons, c->cons_old,
+                           avg, dx, dy, c->prm.yoffset);
+        BX_HIP(hipGetLastError());
+    }
+    auto states = ang ? k_flux_states<true> : k_flux_states<false>;
+    hipLaunchKernelGGL(states, grid, dim3(kThreads), 0, c->stream, c->n, c->nx, c->ny, c->nz, c->flatz, c->cons, c->u,
+                       c->p, c->adindx, c->states, dt, dx, dy, dz, c->prm.small_eps);
     BX_HIP(hipGetLastError());
     const double areaxy = dy * dx, areayz = dy * dz, areaxz = dx * dz, vol = dx * dy * dz;
-    hipLaunchKernelGGL(k_flux_faces, dim3(blocks_for(c->n, kThreads)), dim3(kThreads), 0, c->stream, c->n,
-                       c->nx, c->ny, c->nz, c->flatz, c->states, c->cons, c->cons_old, avg, dt, areayz, areaxz,
-                       areaxy, vol, c->prm.small_rho, c->prm.small_eps);
+    auto faces = ang ? k_flux_faces<true> : k_flux_faces<false>;
+    hipLaunchKernelGGL(faces, grid, dim3(kThreads), 0, c->stream, c->n, c->nx, c->ny, c->nz, c->flatz, c->states,
+                       c->cons, c->cons_old, avg, dt, areayz, areaxz, areaxy, vol, c->prm.small_rho,
+                       c->prm.small_eps, dx, dy, c->prm.yoffset);
     BX_HIP(hipGetLastError());
     return launch_ghosts(c, c->cons);
 }

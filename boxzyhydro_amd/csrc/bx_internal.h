@@ -107,6 +107,10 @@ int hip_check(hipError_t e, const char *what);
         if (_e != hipSuccess) return bx::hip_check(_e, #call);   \
     } while (0)
 
+// interface-state planes of the exact flux: per direction 7 minus + 7 plus
+// (rho, u, v, w, p, eps, gamma), plus r- and angular momentum with flux_ang
+inline int state_planes(const bx_ctx *c) { return c->prm.fluxangmom ? 54 : 42; }
+
 inline unsigned int blocks_for(int64_t n, int threads)
 {
     return (unsigned int)((n + threads - 1) / threads);

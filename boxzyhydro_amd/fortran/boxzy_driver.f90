@@ -107,7 +107,6 @@ program boxzy_driver
   if (command_argument_count() >= 5) call get_command_argument(5, mode)
 
   call read_params(trim(nmlfile))
-  if (fluxangmom) stop 'boxzy_driver: fluxangmom is not on the GPU path'
 
   ! init_grid.f90 counterpart: the context owns the device arrays and tables
   prm%nx = nx; prm%ny = ny; prm%nz = nz
@@ -120,6 +119,8 @@ program boxzy_driver
        merge(1, 0, no_outflow_yl), merge(1, 0, no_outflow_yr), &
        merge(1, 0, no_outflow_zl), merge(1, 0, no_outflow_zr)]
   prm%precision = BX_PREC_FP64
+  prm%fluxangmom = merge(1, 0, fluxangmom)       ! main.f90:343-369 flux_ang, source.f90:73-90
+  prm%yoffset = yoffset
   call bx_check(bx_create(prm, 0_c_int32_t, bxctx), 'create')
   if (H2STAT >= 0) then
      ! main.f90:70-71: initialize_eos + calc_eos_table (tables kept on the device)

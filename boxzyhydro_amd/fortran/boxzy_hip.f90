@@ -26,6 +26,8 @@ module boxzy_hip
      real(c_double) :: avmag(3)
      integer(c_int32_t) :: no_outflow(6)
      integer(c_int32_t) :: precision
+     integer(c_int32_t) :: fluxangmom
+     real(c_double) :: yoffset
   end type bx_params
 
   ! &eos_input keys and the table bounds of parameters.f90:44-58

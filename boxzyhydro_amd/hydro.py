@@ -48,6 +48,8 @@ def make_params(prm: HydroParams, precision: int = nv.BX_PREC_FP64) -> nv.BxPara
                            prm.no_outflow_yr, prm.no_outflow_zl, prm.no_outflow_zr)):
         P.no_outflow[i] = int(bool(v))
     P.precision = precision
+    P.fluxangmom = int(bool(prm.fluxangmom))
+    P.yoffset = prm.yoffset
     return P
 
 

@@ -122,6 +122,28 @@ def box_ic(prm: HydroParams, seed: int = 0, sigma_cells: float = 4.0,
     return _assemble(prm, rho, vel[0], vel[1], vel[2], p)
 
 
+def rotating_box_ic(prm: HydroParams, seed: int = 0, omega: float = 0.2, sigma_cells: float = 2.0,
+                    mach_rms: float = 0.3, yoffset: float = 0.0) -> Dict[str, np.ndarray]:
+    """``box_ic`` plus solid rotation about the z axis through y = -yoffset,
+    u += omega (-(y + yoffset), x, 0): the set-up for flux_ang
+    (fluxangmom=.true.), which fluxes r- and angular momentum about that axis."""
+    base = box_ic(prm, seed=seed, sigma_cells=sigma_cells, mach_rms=mach_rms)
+    x, y, _ = cell_centres(prm)
+    X = np.broadcast_to(x[None, None, :], prm.shape).reshape(-1)
+    Y = np.broadcast_to(y[None, :, None], prm.shape).reshape(-1) + yoffset
+    rho = base["cons"][:, 0].copy()
+    interior = boundary_flags(prm).reshape(-1) != 1
+    rho[~interior] = 1.0
+    v = base["u"].copy()
+    v[:, 0] -= omega * Y
+    v[:, 1] += omega * X
+    if prm.nz == 1:
+        v[:, 2] = 0.0
+    shp = prm.shape
+    return _assemble(prm, rho.reshape(shp), v[:, 0].reshape(shp), v[:, 1].reshape(shp), v[:, 2].reshape(shp),
+                     base["p"].reshape(shp))
+
+
 def h2_cloud_ic(prm: HydroParams, seed: int = 0, sigma_cells: float = 2.0, rho0: float = 0.1,
                 rho_sigma: float = 1.0, t_lo: float = 20.0, t_hi: float = 4000.0,
                 mach_rms: float = 0.5) -> Dict[str, np.ndarray]:

@@ -35,6 +35,8 @@ class BxParams(ctypes.Structure):
         ("avmag", ctypes.c_double * 3),
         ("no_outflow", ctypes.c_int32 * 6),
         ("precision", ctypes.c_int32),
+        ("fluxangmom", ctypes.c_int32),
+        ("yoffset", ctypes.c_double),
     ]
 
 

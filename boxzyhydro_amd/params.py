@@ -31,6 +31,9 @@ class HydroParams:
     no_outflow_yr: bool = True
     no_outflow_zl: bool = True
     no_outflow_zr: bool = True
+    # flux_ang (flux_ang.f90) instead of flux, about the z axis through y = -yoffset
+    fluxangmom: bool = False
+    yoffset: float = 0.0
     # &hydro (input.f90:12-13)
     small_rho: float = 1e-10
     small_eps: float = 1e-40
@@ -107,7 +110,7 @@ class HydroParams:
             "&grid_params",
             f"nx={self.nx}", f"ny={self.ny}", f"nz={self.nz}",
             f"dx={f(self.dx)}", f"dy={f(self.dy)}", f"dz={f(self.dz)}",
-            "yoffset=0d0", "fluxangmom=.false.",
+            f"yoffset={f(self.yoffset)}", f"fluxangmom={b(self.fluxangmom)}",
             f"no_outflow_xl={b(self.no_outflow_xl)}", f"no_outflow_xr={b(self.no_outflow_xr)}",
             f"no_outflow_yl={b(self.no_outflow_yl)}", f"no_outflow_yr={b(self.no_outflow_yr)}",
             f"no_outflow_zl={b(self.no_outflow_zl)}", f"no_outflow_zr={b(self.no_outflow_zr)}",

@@ -47,6 +47,9 @@ typedef struct bx_params {
     double avmag[3];          /* &hydro avmagx, avmagy, avmagz */
     int32_t no_outflow[6];    /* &grid_params no_outflow_{xl,xr,yl,yr,zl,zr} */
     int32_t precision;        /* BX_PREC_FP64 or BX_PREC_MIXED */
+    int32_t fluxangmom;       /* &grid_params: 1 = flux_ang (flux_ang.f90:6-618) and the
+                                 source.f90:73-90 terms, exact call sequence only */
+    double yoffset;           /* &grid_params: y shift of the flux_ang momentum transform */
 } bx_params;
 
 typedef struct bx_ctx bx_ctx;
@@ -80,7 +83,7 @@ int bx_velocity(bx_ctx *ctx, double *maxv);       /* velocity.f90:4-64; maxv may
 int bx_courant(bx_ctx *ctx, double *dt);          /* courant.f90:4-39: dt = cfl/max rate */
 int bx_source(bx_ctx *ctx, double tphase, int32_t siter, double dt); /* source.f90:5-205 */
 int bx_set_ghost_cells(bx_ctx *ctx);              /* utils.f90:305-480                   */
-int bx_flux(bx_ctx *ctx, int32_t avg, double dt); /* flux.f90:6-235                      */
+int bx_flux(bx_ctx *ctx, int32_t avg, double dt); /* flux.f90:6-235; flux_ang.f90:6-281 when fluxangmom */
 int bx_copy_old(bx_ctx *ctx);                     /* main.f90:327-333 cons_old = cons    */
 int bx_avisc(bx_ctx *ctx, double *qq);            /* avisc.f90:4-26; qq(3,ngrid) host or NULL */
 

@@ -34,7 +34,8 @@
 ! arrays, so its output there is undefined and the read can fault.
 !
 ! The step sequence below restates main.f90:211-555 with RADTRAN, COOL,
-! SELFGRAVITY, PARTICLE, EXTERNALGRAV, ENERGYCOR off and fluxangmom=.false.
+! SELFGRAVITY, PARTICLE, EXTERNALGRAV, ENERGYCOR off; flux or flux_ang as
+! the namelist's fluxangmom selects (main.f90:343-369).
 !
 program ref_harness
  use parameters
@@ -178,7 +179,11 @@ program ref_harness
   if(mode==1.and.step==1)call snap(4)
 
   dt=dt*two
-  call flux(0)
+  if(fluxangmom)then   ! main.f90:343-347
+    call flux_ang(0)
+  else
+    call flux(0)
+  endif
   if(mode==1.and.step==1)call snap(5)
 
 !$OMP PARALLEL DEFAULT(SHARED)
@@ -190,7 +195,11 @@ program ref_harness
 !$OMP END PARALLEL
   if(mode==1.and.step==1)call snap(6)
 
-  call flux(1)
+  if(fluxangmom)then   ! main.f90:365-369
+    call flux_ang(1)
+  else
+    call flux(1)
+  endif
   if(mode==1.and.step==1)call snap(7)
 
 !$OMP PARALLEL DEFAULT(SHARED)

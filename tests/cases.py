@@ -136,6 +136,8 @@ def case_inputs(rec: dict):
         ic = stress_ic(prm, gen["seed"])
     elif kind == "eos_cells":
         ic = eos_cells_ic(prm, gen["seed"])
+    elif kind == "rotating_box":
+        ic = ics.rotating_box_ic(prm, seed=gen["seed"], **gen.get("kw", {}))
     elif kind == "h2_cloud":
         ic = ics.h2_cloud_ic(prm, seed=gen["seed"], **gen.get("kw", {}))
     elif kind == "h2_disk":
