@@ -15,8 +15,10 @@
 //
 // Device: state() with the tabulated branch (state.f90:58-79): get_gamma2
 // (eos.f90:653-699) interpolates gamma, mu and T on the (log eps/rho, log rho)
-// grid; p = rho T R / mu; the low-density override.  The tables are packed so
-// that one entry's four values (eng2, gamma2, muc2, tk2) are one 32-byte load.
+// grid; p = rho T R / mu; the low-density override.  Each table entry carries
+// its values and its interpolation slopes (64 bytes), stored beside the entry
+// of the next density row at the same temperature index, so one lookup reads
+// one 128-byte line.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -289,13 +291,29 @@ static int build_tables(const bx_params *p, const bx_eos_params *e, Tables &t)
 }
 
 // ------------------------------------------------------------------ device
+// One entry of table 2 at (itk, irho): the values and, for itk < NEOS, the
+// slopes (v(itk+1)-v(itk))/(eng2(itk+1)-eng2(itk)) that get_gamma2 forms at
+// run time -- the same IEEE divisions of the same table values, done once.
+struct alignas(64) TabEntry {
+    double eng, gam, muc, tk, sg, sm, st, pad;
+};
+// The entries (itk, irho) and (itk, irho+1) side by side in one 128-byte
+// line: a lookup, which needs both rows at the same itk, reads one line.
+struct alignas(128) TabPair {
+    TabEntry lo, hi;
+};
+// Per density row: rho_table(irho), 1/(rho_table(irho+1)-rho_table(irho))
+// (NaN where the reference does not interpolate in rho), log10 of the first
+// energy, deng_eos_array, and the first and last energies of the row.
+struct alignas(64) RhoRow {
+    double r0, inv_dr, leng1, deng, e_first, e_last, pad0, pad1;
+};
+
 struct EosDev {
     int neos, nrho;
     double log_rho_low, drho, density, tk_bgrnd;
-    const double4 *tab;   // (irho, itk): eng2, gamma2, muc2, tk2 (code units)
-    const double *rho;    // rho_table (code units)
-    const double *leng1;  // log10(eng_table2(1,irho))
-    const double *deng;   // deng_eos_array
+    const TabPair *tab;  // (irho, itk), itk fastest (code units); hi = row irho+1
+    const RhoRow *rows;
 };
 
 // eos.f90:653-699
@@ -305,77 +323,86 @@ __device__ __forceinline__ void get_gamma2(const EosDev &e, double eps, double r
     int irho = fort_int((log10(rho * e.density) - e.log_rho_low) / e.drho) + 1;
     irho = irho < e.nrho ? irho : e.nrho;
     if (irho < 1) irho = 1;
-    int irhop = irho + 1;
-    if (irhop > e.nrho) irhop = irho;
-    const double4 *col = e.tab + (int64_t)(irho - 1) * e.neos;
+    const RhoRow R = e.rows[irho - 1];
     double eng = eps / rho;
-    int ientry = fort_int((log10(eng) - e.leng1[irho - 1]) / e.deng[irho - 1]) + 1;
+    int ientry = fort_int((log10(eng) - R.leng1) / R.deng) + 1;
     if (ientry > e.neos - 1) {
         ientry = e.neos - 1;
-        eng = col[e.neos - 1].x;
+        eng = R.e_last;
     }
-    if (ientry < 1 || eng < col[0].x) {
+    if (ientry < 1 || eng < R.e_first) {
         ientry = 1;
-        eng = col[0].x;
+        eng = R.e_first;
     }
-    const double4 a = col[ientry - 1], b = col[ientry];
-    gam = a.y + (b.y - a.y) / (b.x - a.x) * (eng - a.x);
-    m = a.z + (b.z - a.z) / (b.x - a.x) * (eng - a.x);
-    tk = a.w + (b.w - a.w) / (b.x - a.x) * (eng - a.x);
-    const double r0 = e.rho[irho - 1], r1 = e.rho[irhop - 1];
-    if (!(r0 == r1)) {
-        const double4 *colp = e.tab + (int64_t)(irhop - 1) * e.neos;
-        const double4 ap = colp[ientry - 1], bp = colp[ientry];
-        const double gamp = ap.y + (bp.y - ap.y) / (bp.x - ap.x) * (eng - ap.x);
-        const double mp = ap.z + (bp.z - ap.z) / (bp.x - ap.x) * (eng - ap.x);
-        const double tkp = ap.w + (bp.w - ap.w) / (bp.x - ap.x) * (eng - ap.x);
-        const double slope = 1.0 / (r1 - r0) * (rho - r0);
-        gam = gam + (gamp - gam) * slope;
-        m = m + (mp - m) * slope;
-        tk = tk + (tkp - tk) * slope;
-    }
+    const TabPair *pr = e.tab + (int64_t)(irho - 1) * e.neos + (ientry - 1);
+    const TabEntry a = pr->lo, ap = pr->hi;  // hi: row irho+1 (the row itself on the last row)
+    gam = a.gam + a.sg * (eng - a.eng);
+    m = a.muc + a.sm * (eng - a.eng);
+    tk = a.tk + a.st * (eng - a.eng);
+    // rho_table(irho) /= rho_table(irhop): interpolate in rho; written as
+    // selects so that no branch waits on the table load
+    const double gamp = ap.gam + ap.sg * (eng - ap.eng);
+    const double mp = ap.muc + ap.sm * (eng - ap.eng);
+    const double tkp = ap.tk + ap.st * (eng - ap.eng);
+    const double slope = R.inv_dr * (rho - R.r0);
+    const bool interp = R.inv_dr == R.inv_dr;
+    gam = interp ? gam + (gamp - gam) * slope : gam;
+    m = interp ? m + (mp - m) * slope : m;
+    tk = interp ? tk + (tkp - tk) * slope : tk;
 }
 
-// state.f90:30-79, tabulated branch (H2STAT >= 0)
+// state.f90:30-79, tabulated branch (H2STAT >= 0).  Each thread takes CPT
+// cells a block apart (coalesced), with indices clamped for the loads, so the
+// table lookups of its cells are in flight together.
+template <int CPT>
 __global__ __launch_bounds__(256) void k_state_tab(int64_t n, double *__restrict__ cons,
                                                    const double *__restrict__ u, double *__restrict__ p,
                                                    double *__restrict__ ad, double *__restrict__ mucp,
                                                    EosDev e, double gamfix, double seps, double muc,
                                                    double rgas, int disk, int nx, int ny, double dx, double dy)
 {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    const double rho = cons[c], E = cons[4 * n + c];
-    const double u0 = u[c], u1 = u[n + c], u2 = u[2 * n + c];
-    const double ekin = rho * 0.5 * (u0 * u0 + u1 * u1 + u2 * u2);
-    double eps = vmax(E - ekin, seps);
-    double tk, m, g;
-    if (disk) {
-        // nz == 1: the column is a disk of scale height hscale (state.f90:61-69)
-        const int ix = (int)(c % nx) + 1, iy = (int)((c / nx) % ny) + 1;
-        const double x = dx * ((double)ix - ((double)nx + 1.0) / 2.0);
-        const double y = dy * ((double)iy - ((double)ny + 1.0) / 2.0);
-        const double hscale = sqrt((ad[c] - 1.0) * eps / (2.0 * ekin) * (x * x + y * y));
-        get_gamma2(e, eps / (2.0 * hscale), rho / (2.0 * hscale), tk, m, g);
-    } else {
-        get_gamma2(e, eps, rho, tk, m, g);
+    const int64_t base = (int64_t)blockIdx.x * blockDim.x * CPT + threadIdx.x;
+    double rho[CPT], ekin[CPT], tk[CPT], m[CPT], g[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        const int64_t c0 = base + (int64_t)k * blockDim.x, c = c0 < n ? c0 : n - 1;
+        rho[k] = cons[c];
+        const double E = cons[4 * n + c];
+        const double u0 = u[c], u1 = u[n + c], u2 = u[2 * n + c];
+        ekin[k] = rho[k] * 0.5 * (u0 * u0 + u1 * u1 + u2 * u2);
+        const double eps = vmax(E - ekin[k], seps);
+        if (disk) {
+            // nz == 1: the column is a disk of scale height hscale (state.f90:61-69)
+            const int ix = (int)(c % nx) + 1, iy = (int)((c / nx) % ny) + 1;
+            const double x = dx * ((double)ix - ((double)nx + 1.0) / 2.0);
+            const double y = dy * ((double)iy - ((double)ny + 1.0) / 2.0);
+            const double hscale = sqrt((ad[c] - 1.0) * eps / (2.0 * ekin[k]) * (x * x + y * y));
+            get_gamma2(e, eps / (2.0 * hscale), rho[k] / (2.0 * hscale), tk[k], m[k], g[k]);
+        } else {
+            get_gamma2(e, eps, rho[k], tk[k], m[k], g[k]);
+        }
     }
-    ad[c] = g;
-    mucp[c] = m;
-    double pc = rho * tk * rgas / m;
-    if (rho < 1e-4) {
-        pc = rgas * e.tk_bgrnd / muc * rho;
-        cons[4 * n + c] = pc / (gamfix - 1.0) + ekin;
+#pragma unroll
+    for (int k = 0; k < CPT; k++) {
+        const int64_t c = base + (int64_t)k * blockDim.x;
+        if (c >= n) break;
+        ad[c] = g[k];
+        mucp[c] = m[k];
+        double pc = rho[k] * tk[k] * rgas / m[k];
+        if (rho[k] < 1e-4) {
+            pc = rgas * e.tk_bgrnd / muc * rho[k];
+            cons[4 * n + c] = pc / (gamfix - 1.0) + ekin[k];
+        }
+        p[c] = pc;
     }
-    p[c] = pc;
 }
 
 }  // namespace bx
 
 struct bx_eos {
     bx::EosDev dev;
-    double4 *tab;
-    double *rho, *leng1, *deng;
+    bx::TabPair *tab;
+    bx::RhoRow *rows;
     double *muc_plane;
 };
 
@@ -385,7 +412,7 @@ void eos_free(bx_ctx *c)
 {
     if (!c || !c->eos) return;
     bx_eos *q = c->eos;
-    for (void *d : {(void *)q->tab, (void *)q->rho, (void *)q->leng1, (void *)q->deng, (void *)q->muc_plane})
+    for (void *d : {(void *)q->tab, (void *)q->rows, (void *)q->muc_plane})
         if (d) (void)hipFree(d);
     delete q;
     c->eos = nullptr;
@@ -394,7 +421,9 @@ void eos_free(bx_ctx *c)
 int launch_state_tab(bx_ctx *c)
 {
     const bx_eos *q = c->eos;
-    hipLaunchKernelGGL(k_state_tab, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cons, c->u, c->p,
+    constexpr int kCpt = 2;
+    hipLaunchKernelGGL(k_state_tab<kCpt>, dim3(blocks_for(c->n, 256 * kCpt)), dim3(256), 0, c->stream, c->n, c->cons,
+                       c->u, c->p,
                        c->adindx, q->muc_plane, q->dev, c->prm.gammafix, c->prm.small_eps, c->muc, c->rgas,
                        c->nz == 1 ? 1 : 0, c->nx, c->ny, c->prm.dx, c->prm.dy);
     BX_HIP(hipGetLastError());
@@ -480,26 +509,41 @@ extern "C" int bx_eos_init(bx_ctx *c, const bx_eos_params *e)
     if (rc) return rc;
     eos_free(c);
     const size_t N = (size_t)t.neos * t.nrho;
-    std::vector<double4> pk(N);
-    for (size_t k = 0; k < N; k++) pk[k] = double4{t.eng2[k], t.gam2[k], t.muc2[k], t.tk2[k]};
-    std::vector<double> leng1(t.nrho);
-    for (int ir = 0; ir < t.nrho; ir++) leng1[ir] = std::log10(t.eng2[(size_t)ir * t.neos]);
+    auto entry = [&](size_t k, int it) {
+        TabEntry q{t.eng2[k], t.gam2[k], t.muc2[k], t.tk2[k], 0.0, 0.0, 0.0, 0.0};
+        if (it + 1 < t.neos) {  // eos.f90:684-689 slopes
+            const double de = t.eng2[k + 1] - t.eng2[k];
+            q.sg = (t.gam2[k + 1] - t.gam2[k]) / de;
+            q.sm = (t.muc2[k + 1] - t.muc2[k]) / de;
+            q.st = (t.tk2[k + 1] - t.tk2[k]) / de;
+        }
+        return q;
+    };
+    std::vector<TabPair> pk(N);
+    parallel_for(t.nrho, [&](int ir) {
+        const int irp = ir + 1 < t.nrho ? ir + 1 : ir;
+        for (int it = 0; it < t.neos; it++)
+            pk[(size_t)ir * t.neos + it] = TabPair{entry((size_t)ir * t.neos + it, it), entry((size_t)irp * t.neos + it, it)};
+    });
+    std::vector<RhoRow> rows(t.nrho);
+    for (int ir = 0; ir < t.nrho; ir++) {
+        const int irp = ir + 1 < t.nrho ? ir + 1 : ir;  // eos.f90:659-660
+        const double r0 = t.rho[ir], r1 = t.rho[irp];
+        const size_t o = (size_t)ir * t.neos;
+        rows[ir] = RhoRow{r0, r0 == r1 ? std::nan("") : 1.0 / (r1 - r0), std::log10(t.eng2[o]), t.deng[ir],
+                          t.eng2[o], t.eng2[o + t.neos - 1], 0.0, 0.0};
+    }
     bx_eos *q = new bx_eos();
     c->eos = q;
-    BX_HIP(hipMalloc(&q->tab, N * sizeof(double4)));
-    BX_HIP(hipMalloc(&q->rho, t.nrho * sizeof(double)));
-    BX_HIP(hipMalloc(&q->leng1, t.nrho * sizeof(double)));
-    BX_HIP(hipMalloc(&q->deng, t.nrho * sizeof(double)));
+    BX_HIP(hipMalloc(&q->tab, N * sizeof(TabPair)));
+    BX_HIP(hipMalloc(&q->rows, t.nrho * sizeof(RhoRow)));
     BX_HIP(hipMalloc(&q->muc_plane, c->n * sizeof(double)));
-    BX_HIP(hipMemcpy(q->tab, pk.data(), N * sizeof(double4), hipMemcpyHostToDevice));
-    BX_HIP(hipMemcpy(q->rho, t.rho.data(), t.nrho * sizeof(double), hipMemcpyHostToDevice));
-    BX_HIP(hipMemcpy(q->leng1, leng1.data(), t.nrho * sizeof(double), hipMemcpyHostToDevice));
-    BX_HIP(hipMemcpy(q->deng, t.deng.data(), t.nrho * sizeof(double), hipMemcpyHostToDevice));
+    BX_HIP(hipMemcpy(q->tab, pk.data(), N * sizeof(TabPair), hipMemcpyHostToDevice));
+    BX_HIP(hipMemcpy(q->rows, rows.data(), t.nrho * sizeof(RhoRow), hipMemcpyHostToDevice));
     // init_grid.f90:294: muc_array starts at zero
     BX_HIP(hipMemset(q->muc_plane, 0, c->n * sizeof(double)));
     const Units scl = get_units();
-    q->dev = EosDev{t.neos, t.nrho, t.log_rho_low, t.drho, scl.density, e->tk_bgrnd,
-                    q->tab, q->rho, q->leng1, q->deng};
+    q->dev = EosDev{t.neos, t.nrho, t.log_rho_low, t.drho, scl.density, e->tk_bgrnd, q->tab, q->rows};
     // the fused kernels assume a fixed gamma: bx_step's fused mode runs exact steps
     c->fused_unsupported = true;
     return BX_OK;
