@@ -311,6 +311,7 @@ int bx_destroy(bx_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     adjoint_free(c);
     eos_free(c);
+    gravity_free(c);
     void *dev[] = {c->cons, c->cons_old, c->u, c->p, c->adindx, c->qq, c->states, c->ghost_dst, c->ghost_src,
                    c->ghost_sign, c->ghost_scratch_src, c->ghost_scratch, c->ustar, c->pstar, c->unext, c->dsc,
                    c->cons32, c->ustar32, c->unext32, c->pstar32, c->eflux};
@@ -591,6 +592,8 @@ static int exact_step(bx_ctx *c, double endtime, double *time, double *dtold, in
     if ((rc = launch_state(c))) return rc;
     if ((rc = launch_velocity(c, false))) return rc;
     dt = 0.5 * dt;
+    if ((c->grav_flags & BX_GRAV_POTENTIAL) && (rc = launch_calc_gforce(c))) return rc;   // :481
+    if ((c->grav_flags & BX_GRAV_EXTERNAL) && (rc = launch_external_grav(c))) return rc;  // :487
     if ((rc = launch_source(c, dt))) return rc;  // :490 (+ ghosts, source.f90:199)
     if ((rc = ghosts_again(c))) return rc;       // :511
     if ((rc = launch_state(c))) return rc;

@@ -133,15 +133,17 @@ __global__ __launch_bounds__(kThreads) void k_courant(int64_t n, const double *_
 __device__ __forceinline__ double centre(double d, int i, int n) { return d * ((double)i - ((double)n + 1.0) / 2.0); }
 
 // ----------------------------------------------------------------- source
-// source.f90:148-182 on interior cells with a uniform gravity vector g
-// (the reference's g = -(phi(+)-phi(-))/(2d) = -0.0 for phi=0); with
+// source.f90:148-182 on interior cells with the gforce planes gf, or without
+// them a uniform gravity vector g (the reference's g = -(phi(+)-phi(-))/(2d)
+// = -0.0 for phi=0); with
 // fluxangmom the terms of source.f90:73-90 come first (the pforce of
 // source.f90:64-66 feeds nothing on the hydro path)
 __global__ __launch_bounds__(kThreads) void k_source(int64_t n, int nx, int ny, int nz,
                                                     double *__restrict__ cons,
                                                     const double *__restrict__ u, double dt,
                                                     double g0, double g1, double g2, double seps,
-                                                    int ang, double dx, double dy)
+                                                    int ang, double dx, double dy,
+                                                    const double *__restrict__ gf)
 {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
@@ -158,6 +160,11 @@ __global__ __launch_bounds__(kThreads) void k_source(int64_t n, int nx, int ny, 
         const double w = x * uy - y * ux, r4 = ((r * r) * r) * r;
         m1 = m1 + rho * (w * w) * y / r4 * dt;
         m0 = m0 + rho * (w * w) * x / r4 * dt;
+    }
+    if (gf) {  // gforce(1:3, igrid) of calc_gforce / external_grav
+        g0 = gf[c];
+        g1 = gf[n + c];
+        g2 = gf[2 * n + c];
     }
     m1 = m1 + (rho * g1) * dt;
     m0 = m0 + (rho * g0) * dt;
@@ -564,7 +571,7 @@ int launch_source(bx_ctx *c, double dt)
 {
     hipLaunchKernelGGL(k_source, dim3(blocks_for(c->n, kThreads)), dim3(kThreads), 0, c->stream, c->n, c->nx,
                        c->ny, c->nz, c->cons, c->u, dt, c->grav[0], c->grav[1], c->grav[2], c->prm.small_eps,
-                       c->prm.fluxangmom, c->prm.dx, c->prm.dy);
+                       c->prm.fluxangmom, c->prm.dx, c->prm.dy, c->grav_flags ? c->gforce : (const double *)nullptr);
     BX_HIP(hipGetLastError());
     return launch_ghosts(c, c->cons);
 }

@@ -94,6 +94,13 @@ struct bx_ctx {
 
     struct bx_adj *adj;  // adjoint store (bx_adjoint.hip), or null
     struct bx_eos *eos;  // tabulated EOS (bx_eos.hip), or null for a fixed gamma
+
+    // gravity (bx_gravity.hip): a given potential and per-cell gforce(3,n) planes
+    int grav_flags;                // BX_GRAV_* bits, 0 = uniform -0.0 force of phi = 0
+    double obj_mass, obj_radius;   // external_grav point mass (parameters.f90:83-86)
+    double *phi;                   // n
+    double *gforce;                // 3 planes
+    double *grav_sums;             // centre-of-mass sums (3) and tree partials
 };
 
 namespace bx {
@@ -132,6 +139,9 @@ int fused_step(bx_ctx *c, double dt);
 int fused_step_into(bx_ctx *c, double dt, double *u0, double *us, double *ps, double *un, bool reset_flags);
 void adjoint_free(bx_ctx *c);
 void eos_free(bx_ctx *c);
+void gravity_free(bx_ctx *c);
+int launch_calc_gforce(bx_ctx *c);
+int launch_external_grav(bx_ctx *c);
 int launch_state_tab(bx_ctx *c);
 int fused_prepare(bx_ctx *c);
 int narrow_cons(bx_ctx *c);

@@ -138,6 +138,43 @@ class HydroGrid:
                                       nv.dptr(out["p"]), nv.dptr(out["adindx"])), "bx_download")
         return out
 
+    # -------------------------------------------------------------- gravity
+    def set_gravity(self, flags: int, phi: Optional[np.ndarray] = None, object_mass: float = 1.0,
+                    object_radius: float = 0.625, initial: bool = True) -> None:
+        """Gravity from a given potential ``phi`` (ngrid): BX_GRAV_POTENTIAL
+        (calc_gforce), BX_GRAV_EXTERNAL (external_grav's point mass),
+        BX_GRAV_COM_TREE (tree-ordered mass moments).  ``initial`` makes the
+        calls of main.f90:146-154 that precede the step loop."""
+        self.grav_flags = flags
+        nv.check(nv.lib().bx_gravity_config(self._h, flags, object_mass, object_radius), "bx_gravity_config")
+        if not flags:
+            return
+        if phi is not None:
+            phi = np.ascontiguousarray(phi, np.float64)
+            if phi.size != self.ngrid:
+                raise ValueError(f"phi: expected {self.ngrid} values, got {phi.size}")
+        nv.check(nv.lib().bx_set_phi(self._h, nv.dptr(phi)), "bx_set_phi")
+        if initial:
+            if flags & nv.BX_GRAV_POTENTIAL:
+                self.calc_gforce()
+            if flags & nv.BX_GRAV_EXTERNAL:
+                self.external_grav()
+
+    def calc_gforce(self) -> None:
+        """utils.f90:549-571 on the device."""
+        nv.check(nv.lib().bx_calc_gforce(self._h), "bx_calc_gforce")
+
+    def external_grav(self) -> np.ndarray:
+        """gravity.f90:1477-1549; returns the mass moments (x, y, z)."""
+        com = np.empty(3)
+        nv.check(nv.lib().bx_external_grav(self._h, nv.dptr(com)), "bx_external_grav")
+        return com
+
+    def download_gforce(self) -> np.ndarray:
+        out = np.empty((self.ngrid, 3))
+        nv.check(nv.lib().bx_download_gforce(self._h, nv.dptr(out)), "bx_download_gforce")
+        return out
+
     def download_muc(self) -> np.ndarray:
         """muc_array (grid_commons.f90:12) as the last state() left it."""
         out = np.empty(self.ngrid)

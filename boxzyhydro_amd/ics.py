@@ -274,9 +274,12 @@ def box_workload(n_interior: int, seed: int = 0):
 
 
 def write_ic_file(path: str, ic: Dict[str, np.ndarray]) -> None:
-    """Stream-binary IC file read by the reference driver: cons u p adindx."""
+    """Stream-binary IC file read by the reference driver: cons u p adindx,
+    then phi when ``ic`` has one."""
     with open(path, "wb") as f:
-        for k in ("cons", "u", "p", "adindx"):
+        for k in ("cons", "u", "p", "adindx", "phi"):
+            if k == "phi" and k not in ic:
+                continue
             f.write(np.ascontiguousarray(ic[k], np.float64).tobytes())
 
 

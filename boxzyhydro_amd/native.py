@@ -19,6 +19,9 @@ BX_PREC_FP64 = 0
 BX_PREC_MIXED = 1
 BX_STEP_EXACT = 0
 BX_STEP_FUSED = 1
+BX_GRAV_POTENTIAL = 1
+BX_GRAV_EXTERNAL = 2
+BX_GRAV_COM_TREE = 4
 
 
 class BxParams(ctypes.Structure):
@@ -112,6 +115,11 @@ _SIGS = {
     "bx_eos_tables": [ctypes.POINTER(BxParams), ctypes.POINTER(BxEosParams)] + [_DP] * 11,
     "bx_eos_init": [_VP, ctypes.POINTER(BxEosParams)],
     "bx_download_muc": [_VP, _DP],
+    "bx_gravity_config": [_VP, _I32, ctypes.c_double, ctypes.c_double],
+    "bx_set_phi": [_VP, _DP],
+    "bx_calc_gforce": [_VP],
+    "bx_external_grav": [_VP, _DP],
+    "bx_download_gforce": [_VP, _DP],
 }
 
 EXPORTED = tuple(_SIGS) + ("bx_last_error", "bx_eos_defaults")

@@ -271,6 +271,34 @@ int bx_eos_init(bx_ctx *ctx, const bx_eos_params *e);
 /* muc_array(ngrid) (grid_commons.f90:12) as the last state() left it. */
 int bx_download_muc(bx_ctx *ctx, double *muc_array);
 
+/* ------------------------------------------------------------------------
+ * Gravity from a given potential.  The reference computes gforce from the
+ * potential phi (calc_gforce, utils.f90:549-571: centred differences on
+ * interior cells) and adds an optional softened point mass placed at minus
+ * the grid's mass moment (external_grav, gravity.f90:1477-1549); source()
+ * applies gforce (source.f90:148-182).  phi itself comes from its
+ * self-gravity solver, which is not part of this library: here it is given.
+ * With flags set, bx_step makes the main-loop calls (main.f90:479-487:
+ * calc_gforce, then external_grav, before source(tphase,1)) and runs the
+ * exact sequence in either mode.  The caller makes the calls before the
+ * loop (main.f90:146-154) with bx_calc_gforce / bx_external_grav.
+ */
+#define BX_GRAV_POTENTIAL 1 /* calc_gforce from phi (USEPOTENTIAL) */
+#define BX_GRAV_EXTERNAL 2  /* external_grav point mass (EXTERNALGRAV) */
+#define BX_GRAV_COM_TREE 4  /* mass moment by a fixed-order tree sum; without it, a
+                               sequential sum in cell order, bit-identical to the
+                               reference's reduction on one OpenMP thread */
+/* flags 0 restores the zero-potential force.  object_mass and object_radius
+ * are parameters.f90:83-86 (defaults 1.0 and 0.625).  FP64 contexts, before
+ * the first fused step.  Sets phi = 0 and gforce = 0 (init_grid.f90:291-299). */
+int bx_gravity_config(bx_ctx *ctx, int32_t flags, double object_mass, double object_radius);
+int bx_set_phi(bx_ctx *ctx, const double *phi);   /* phi(ngrid), host */
+int bx_calc_gforce(bx_ctx *ctx);                  /* utils.f90:549-571 */
+/* gravity.f90:1477-1549; com (may be NULL) receives the mass moments
+ * sum x rho dx dy dz, sum y ..., sum z ...  Synchronises when com is given. */
+int bx_external_grav(bx_ctx *ctx, double *com);
+int bx_download_gforce(bx_ctx *ctx, double *gforce); /* gforce(3,ngrid) */
+
 /* Synchronise the context's stream. */
 int bx_sync(bx_ctx *ctx);
 

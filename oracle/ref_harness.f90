@@ -8,7 +8,7 @@
 ! It is never shipped and never called by the product path; its outputs are
 ! golden vectors for the C restatement (boxzy_oracle.c) and the HIP kernels.
 !
-! Usage:  ref_harness <namelist> <ic_file> <out_prefix> <max_steps> <mode> [diag] [dump]
+! Usage:  ref_harness <namelist> <ic_file> <out_prefix> <max_steps> <mode> [diag] [dump] [grav]
 !   ic_file   stream-binary float64: cons(5,ngrid) u(3,ngrid) p(ngrid) adindx(ngrid)
 !             (Fortran order, exactly the reference's grid_commons arrays)
 !   mode 0    run steps; write <prefix>.dt (dt, time per step),
@@ -28,6 +28,12 @@
 !   diag 1    also run the per-step conservation diagnostics (main.f90:222-223)
 !   dump 1|2  after the run, write_files(step) with write_ascii .false.|.true.
 !             (celldump.<step> in the working directory)
+!   grav 1    the potential phi(ngrid) is a fifth record of ic_file; gforce
+!             from calc_gforce() before the loop and before source(tphase,1)
+!             of every step (main.f90:146-148, 479-481, USEPOTENTIAL with a
+!             fixed potential: no Poisson solve)
+!   grav 2    as 1, and external_grav() after each calc_gforce()
+!             (main.f90:152-154, 485-487, EXTERNALGRAV)
 !
 ! avisc (avisc.f90:4-26) is not driven here: for outer-layer ghosts it reads
 ! grid(0) and u(:,0) (ineigh(2:6)=0 from init_grid.f90:178), outside the
@@ -44,12 +50,13 @@ program ref_harness
  use eos
  use input
  use utils
+ use selfgravity, only: external_grav
  use omp_lib
  implicit none
 
  character(len=256)::nmlfile,icfile,prefix,arg
  character(len=80)::nml80
- integer::maxsteps,mode,diag,dump,step,igrid,idim,ios
+ integer::maxsteps,mode,diag,dump,grav,step,igrid,idim,ios
  real(pre)::f1,dtold,tphase
  real(pre)::etot_loc0,mass_loc0,f_loc0
  real(8)::wt0,wt1
@@ -67,6 +74,10 @@ program ref_harness
  dump=0
  if(command_argument_count()>=7)then
    call get_command_argument(7,arg); read(arg,*)dump
+ endif
+ grav=0
+ if(command_argument_count()>=8)then
+   call get_command_argument(8,arg); read(arg,*)grav
  endif
 
  nml80=nmlfile(1:80)
@@ -100,10 +111,15 @@ program ref_harness
  read(201)u
  read(201)p
  read(201)adindx
- close(201)
  phi=zero
+ if(grav>0)read(201)phi
+ close(201)
 !$OMP PARALLEL DEFAULT(SHARED)
  call calc_gforce()
+ if(grav==2)then
+!$OMP BARRIER
+   call external_grav(zero,0)
+ endif
 !$OMP END PARALLEL
  cons_old=cons
 
@@ -213,7 +229,13 @@ program ref_harness
 
   dt=half*dt
 !$OMP PARALLEL DEFAULT(SHARED)
+  if(grav>0)call calc_gforce()
   tphase=time+dt
+  if(grav==2)then
+!$OMP BARRIER
+    call external_grav(tphase,0)
+  endif
+!$OMP BARRIER
   call source(tphase,1)
   call set_ghost_cells()
 !$OMP BARRIER

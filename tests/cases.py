@@ -72,6 +72,26 @@ def eos_cells_ic(prm: HydroParams, seed: int):
             "adindx": rng.uniform(1.1, 1.67, n)}
 
 
+def gravity_ic(prm: HydroParams, gen: dict):
+    """A seeded set-up with a fixed potential: the rotating box of
+    ``ics.rotating_box_ic`` (or the H2 disk when ``gen['h2']``) plus phi, a
+    softened point-mass potential -m / sqrt(r^2 + a^2) about (x0, y0) with a
+    seeded ripple (enters calc_gforce, utils.f90:549-571)."""
+    base = ics.h2_disk_ic(prm, seed=gen["seed"]) if gen.get("h2") else \
+        ics.rotating_box_ic(prm, seed=gen["seed"], **gen.get("kw", {}))
+    x, y, z = ics.cell_centres(prm)
+    X = np.broadcast_to(x[None, None, :], prm.shape)
+    Y = np.broadcast_to(y[None, :, None], prm.shape)
+    Z = np.broadcast_to(z[:, None, None], prm.shape)
+    m, a = gen.get("phi_mass", 1.0), gen.get("phi_soft", 0.5)
+    x0, y0 = gen.get("phi_x0", 0.13), gen.get("phi_y0", -0.07)
+    rng = np.random.Generator(np.random.PCG64(gen["seed"] + 1000))
+    ripple = 0.02 * rng.standard_normal(prm.shape)
+    phi = -m / np.sqrt((X - x0) ** 2 + (Y - y0) ** 2 + Z ** 2 + a * a) + ripple
+    base["phi"] = np.ascontiguousarray(phi.reshape(-1))
+    return base
+
+
 def stress_tubes():
     """Shock tubes that take the exact branches of the step: a vacuum-forming
     rarefaction (state.f90:52-56 low-density branch, density and energy
@@ -136,6 +156,8 @@ def case_inputs(rec: dict):
         ic = stress_ic(prm, gen["seed"])
     elif kind == "eos_cells":
         ic = eos_cells_ic(prm, gen["seed"])
+    elif kind == "gravity":
+        ic = gravity_ic(prm, gen)
     elif kind == "rotating_box":
         ic = ics.rotating_box_ic(prm, seed=gen["seed"], **gen.get("kw", {}))
     elif kind == "h2_cloud":
@@ -210,6 +232,9 @@ class Driver:
             if s:
                 s(8, dt)
             dt = 0.5 * dt
+            gravity = getattr(g, "gravity", None)
+            if gravity:  # main.f90:479-487: calc_gforce, external_grav
+                gravity()
             g.source(dt)
             g.set_ghost_cells()
             g.state()
@@ -261,6 +286,13 @@ class HipAdapter:
     def flux(self, avg, dt):
         self.hg.dt = dt
         self.hg.flux(avg)
+
+    def gravity(self):
+        flags = getattr(self.hg, "grav_flags", 0)
+        if flags & 1:
+            self.hg.calc_gforce()
+        if flags & 2:
+            self.hg.external_grav()
 
     def arrays(self) -> Dict[str, np.ndarray]:
         d = self.hg.download()

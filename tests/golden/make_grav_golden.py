@@ -1,0 +1,78 @@
+"""Write the golden vectors of gravity from a given potential from the
+reference's own code (oracle/_ref/ref_harness with grav=1: calc_gforce from a
+fixed phi before the loop and before source(tphase,1) of every step, as
+main.f90:146-148 and 479-481 do with USEPOTENTIAL; grav=2: external_grav after
+each calc_gforce, main.f90:152-154 and 485-487 with EXTERNALGRAV).
+external_grav's mass moments are an OpenMP reduction, sequential only on one
+thread: those cases are generated with one thread.
+
+Only run in the build container (the reference is not available elsewhere):
+
+    python tests/golden/make_grav_golden.py
+"""
+from __future__ import annotations
+
+import dataclasses
+import json
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from boxzyhydro_amd.params import HydroParams  # noqa: E402
+from cases import case_inputs, ic_digest, sha  # noqa: E402
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "grav.json")
+OUTFLOW = dict(no_outflow_xl=False, no_outflow_xr=False, no_outflow_yl=False, no_outflow_yr=False,
+               no_outflow_zl=False, no_outflow_zr=False)
+
+
+def record(prm, gen, steps, threads, mode, grav):
+    import hashlib
+
+    import numpy as np
+    import pyoracle as po
+
+    rec = {"params": {k: getattr(prm, k) for k in prm.__dataclass_fields__}, "generator": gen}
+    _, ic = case_inputs(rec)
+    r = po.run_reference(prm, ic, steps, mode=mode, threads=threads, grav=grav)
+    rec.update({"grav": grav, "ic_sha256": ic_digest(ic),
+                "phi_sha256": hashlib.sha256(np.ascontiguousarray(ic["phi"], "<f8").tobytes()).hexdigest(),
+                "steps": int(len(r["dt"])), "threads": threads,
+                "dt": [float(x) for x in r["dt"]], "time": [float(x) for x in r["time"]],
+                "final_sha256": {k: sha(r[k]) for k in ("cons", "u", "p")}})
+    if mode == 1:
+        rec["snaps"] = [{"tag": s["tag"], "dt": s["dt"],
+                         "sha256": {k: sha(s[k]) for k in ("cons", "u", "p", "cons_old")}} for s in r["snaps"]]
+    return rec
+
+
+def main() -> None:
+    import pyoracle as po
+
+    if not po.have_reference():
+        raise SystemExit("build the reference driver first: make -C oracle ref")
+    cases = {}
+    box = HydroParams(nx=16, ny=16, nz=14, dx=0.15, dy=0.15, dz=0.15, endtime=1e30, cfl=0.4, gammafix=5.0 / 3.0,
+                      **OUTFLOW)
+    cases["grav_pot_box"] = record(box, {"kind": "gravity", "seed": 1, "kw": {"omega": 0.5}}, 10, 4, 1, 1)
+    disk = HydroParams(nx=28, ny=28, nz=1, dx=0.1, dy=0.1, dz=0.1, endtime=1e30, cfl=0.4, gammafix=1.4, **OUTFLOW)
+    cases["grav_ext_disk"] = record(disk, {"kind": "gravity", "seed": 2, "phi_mass": 0.5,
+                                           "kw": {"omega": 0.8}}, 10, 1, 1, 2)
+    cases["grav_ext_ang_disk"] = record(dataclasses.replace(disk, fluxangmom=True),
+                                        {"kind": "gravity", "seed": 3, "phi_mass": 0.5, "kw": {"omega": 0.8}},
+                                        10, 1, 0, 2)
+    walls = HydroParams(nx=14, ny=16, nz=10, dx=0.2, dy=0.2, dz=0.2, endtime=1e30, cfl=0.4, gammafix=1.4)
+    cases["grav_ext_walls3d"] = record(walls, {"kind": "gravity", "seed": 4, "kw": {"omega": 0.3}}, 8, 1, 0, 2)
+    with open(OUT, "w") as f:
+        json.dump(cases, f, indent=1)
+    for k, r in cases.items():
+        print(k, r["steps"], r["dt"][:3])
+    print("wrote", OUT)
+
+
+if __name__ == "__main__":
+    main()
