@@ -144,7 +144,30 @@ module boxzy_hip
        import :: c_int, c_ptr
        type(c_ptr), value :: ctx
      end function
+     integer(c_int) function bx_gravity_config(ctx, flags, object_mass, object_radius) &
+          bind(C, name='bx_gravity_config')
+       import :: c_int, c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: flags
+       real(c_double), value :: object_mass, object_radius
+     end function
+     integer(c_int) function bx_set_phi(ctx, phi) bind(C, name='bx_set_phi')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: phi(*)
+     end function
+     integer(c_int) function bx_calc_gforce_c(ctx) bind(C, name='bx_calc_gforce')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function
+     integer(c_int) function bx_external_grav_c(ctx, com) bind(C, name='bx_external_grav')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: com(3)
+     end function
   end interface
+
+  integer, parameter :: BX_GRAV_POTENTIAL = 1, BX_GRAV_EXTERNAL = 2, BX_GRAV_COM_TREE = 4
 
   ! the device counterpart of grid_commons: context plus the step scalars
   type(c_ptr), save :: bxctx = c_null_ptr
@@ -203,6 +226,22 @@ contains
 
   ! write_files.f90:5-71 (hydro part): state(), then celldump.<step> (binary,
   ! or formatted when ascii); time is the module variable
+  ! utils.f90:549-571 on the device (gravity configured with bx_gravity_config
+  ! and the potential given with bx_set_phi)
+  subroutine calc_gforce()
+    call bx_check(bx_calc_gforce_c(bxctx), 'calc_gforce')
+  end subroutine
+
+  ! gravity.f90:1477-1549; object_mass as given to bx_gravity_config
+  subroutine external_grav(tphase, printcom, object_mass)
+    real(c_double), intent(in) :: tphase, object_mass
+    integer, intent(in) :: printcom
+    real(c_double) :: com(3)
+    call bx_check(bx_external_grav_c(bxctx, com), 'external_grav')
+    if (printcom == 1) print *, "STAR ", tphase, -com(1) / object_mass, -com(2) / object_mass, &
+         -com(3) / object_mass
+  end subroutine
+
   subroutine write_files(step, ascii)
     integer, intent(in) :: step
     logical, intent(in) :: ascii
