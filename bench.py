@@ -13,6 +13,7 @@ the dominant kernel (the corrector's x/z kernel) timed with HIP events on the
 library's stream, the mixed-precision form on the same box, the batched
 shock tubes (512 problems, one persistent launch replayed from a hipGraph),
 the tabulated H2 EOS (state() kernel rate, step cost against a fixed gamma),
+flux_ang and gravity from a given potential (step costs),
 the adjoint/forward cost (128^3 box, 200 steps, checkpoints every 20) and a
 CPU baseline: the reference's own Fortran subroutines (oracle/_ref) timed on
 the host on a bounded sample, or the C restatement when that build is absent.
@@ -272,6 +273,56 @@ def eos_cost(n: int, steps: int, reps: int):
             "tabulated_over_fixed_step_time": t_tab / t_fix}
 
 
+def options_cost(n: int, steps: int):
+    """Cost of flux_ang and of gravity from a given potential on an n^3
+    rotating box: exact-sequence step times against the plain exact step,
+    and the mass-moment reduction of external_grav in both orders."""
+    import dataclasses
+
+    import numpy as np
+
+    from boxzyhydro_amd import ics
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+    from boxzyhydro_amd.params import box_params
+
+    prm = box_params(n)
+    ic = ics.rotating_box_ic(prm, seed=0, omega=0.002)
+    x, y, z = ics.cell_centres(prm)
+    phi = (-1.0 / np.sqrt(x[None, None, :] ** 2 + y[None, :, None] ** 2 + z[:, None, None] ** 2 + 100.0)).reshape(-1)
+
+    def step_ms(p, grav=0):
+        g = HydroGrid(p, ic, device=0)
+        if grav:
+            g.set_gravity(grav, phi, object_mass=1.0, object_radius=20.0)
+        g.run(1, mode=nv.BX_STEP_EXACT)
+        g.sync()
+        t0 = time.perf_counter()
+        g.run(steps, mode=nv.BX_STEP_EXACT)
+        g.sync()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        moments = None
+        if grav & nv.BX_GRAV_EXTERNAL:
+            g.external_grav()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                g.external_grav()
+            moments = (time.perf_counter() - t0) / 3 * 1e3
+        g.close()
+        return ms, moments
+
+    base, _ = step_ms(prm)
+    ang, _ = step_ms(dataclasses.replace(prm, fluxangmom=True))
+    tree, tree_call = step_ms(prm, nv.BX_GRAV_POTENTIAL | nv.BX_GRAV_EXTERNAL)
+    seq, seq_call = step_ms(prm, nv.BX_GRAV_POTENTIAL | nv.BX_GRAV_EXTERNAL | nv.BX_GRAV_COM_SEQUENTIAL)
+    return {"workload": f"rotating box {n}^3 interior FP64, exact call sequence, {steps} steps each",
+            "exact_step_ms": base, "flux_ang_step_ms": ang, "flux_ang_over_plain": ang / base,
+            "gravity_tree_step_ms": tree, "gravity_sequential_step_ms": seq,
+            "external_grav_call_ms": {"tree": tree_call, "sequential": seq_call},
+            "note": "gravity = calc_gforce + external_grav before source(tphase,1) each step; the sequential "
+                    "mass moments (bit-identical to a one-thread reference) run on one wave"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,7 +338,9 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched shock tubes")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost")
     ap.add_argument("--no-eos", action="store_true", help="skip the tabulated-EOS line")
+    ap.add_argument("--no-options", action="store_true", help="skip the flux_ang / gravity line")
     ap.add_argument("--eos-only", action="store_true", help="run only the tabulated-EOS line")
+    ap.add_argument("--options-only", action="store_true", help="run only the flux_ang / gravity line")
     ap.add_argument("--adjoint-size", type=int, default=128)
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--mixed-study", action="store_true",
@@ -320,6 +373,9 @@ def main() -> int:
 
     if args.eos_only:
         print(json.dumps({"eos": eos_cost(args.size, 5, 20)}))
+        return 0
+    if args.options_only:
+        print(json.dumps({"options": options_cost(args.size, 3)}))
         return 0
     prm = box_params(args.size)
     log(f"generating box {args.size}^3 ...")
@@ -389,6 +445,9 @@ def main() -> int:
     if not args.no_eos:
         log("tabulated EOS ...")
         out["eos"] = eos_cost(args.size, 5, 20)
+    if not args.no_options:
+        log("flux_ang and gravity ...")
+        out["options"] = options_cost(args.size, 3)
     if not args.no_adjoint:
         log(f"adjoint {args.adjoint_size}^3 x {args.adjoint_steps} ...")
         out["adjoint"] = adjoint_cost(args.adjoint_size, args.adjoint_steps, 20)

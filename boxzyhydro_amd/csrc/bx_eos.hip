@@ -502,6 +502,7 @@ extern "C" int bx_eos_init(bx_ctx *c, const bx_eos_params *e)
     BX_HIP(hipSetDevice(c->device));
     if (e->H2STAT == -1) {  // fixed gamma: state() needs no table (state.f90:49-57)
         eos_free(c);
+        c->fused_unsupported = false;  // fused_prepare decides again on the next fused step
         return BX_OK;
     }
     Tables t;

@@ -488,6 +488,7 @@ int fused_prepare(bx_ctx *c)
         return set_error(BX_ERR_UNSUPPORTED, "fused step assumes zero gravity");
     if (c->prm.fluxangmom) return set_error(BX_ERR_UNSUPPORTED, "fused step fluxes linear momentum (fluxangmom off)");
     if (c->grav_flags) return set_error(BX_ERR_UNSUPPORTED, "fused step assumes the zero-potential force");
+    if (c->eos) return set_error(BX_ERR_UNSUPPORTED, "fused step assumes a fixed gamma (H2STAT=-1)");
     if (c->prm.precision == BX_PREC_MIXED) {
         const size_t pf = sizeof(float) * (size_t)c->n;
         if (!c->cons32) BX_HIP(hipMalloc((void **)&c->cons32, 5 * pf));

@@ -47,23 +47,37 @@ __device__ __forceinline__ void moment_terms(int64_t c, int nx, int ny, int nz, 
 }
 
 // The mass moments summed in cell order, as the reference's reduction does on
-// one OpenMP thread: one wave loads 64 cells at a time and folds them into the
-// running sums lane by lane.  The private sum starts at +0 and is added to the
-// shared zero at the end (OpenMP reduction semantics).
+// one OpenMP thread: one wave computes the terms of 64 cells at a time and
+// folds them into the running sums lane by lane (readlane, no LDS round trip),
+// while the next 64 cells load.  The private sum starts at +0 and is added to
+// the shared zero at the end (OpenMP reduction semantics).
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __global__ __launch_bounds__(64) void k_moments_seq(int64_t n, int nx, int ny, int nz, const double *__restrict__ rho,
                                                    double dx, double dy, double dz, double *__restrict__ sums)
 {
     const int lane = threadIdx.x;
     double sx = 0.0, sy = 0.0, sz = 0.0;
+    double tx = 0.0, ty = 0.0, tz = 0.0;
+    if (lane < n) moment_terms(lane, nx, ny, nz, rho, dx, dy, dz, tx, ty, tz);
     for (int64_t b = 0; b < n; b += 64) {
-        double tx = 0.0, ty = 0.0, tz = 0.0;
-        if (b + lane < n) moment_terms(b + lane, nx, ny, nz, rho, dx, dy, dz, tx, ty, tz);
+        double nxt[3] = {0.0, 0.0, 0.0};
+        if (b + 64 + lane < n) moment_terms(b + 64 + lane, nx, ny, nz, rho, dx, dy, dz, nxt[0], nxt[1], nxt[2]);
         const int m = n - b < 64 ? (int)(n - b) : 64;
         for (int l = 0; l < m; l++) {
-            sx = sx + __shfl(tx, l, 64);
-            sy = sy + __shfl(ty, l, 64);
-            sz = sz + __shfl(tz, l, 64);
+            sx = sx + lane_value(tx, l);
+            sy = sy + lane_value(ty, l);
+            sz = sz + lane_value(tz, l);
         }
+        tx = nxt[0];
+        ty = nxt[1];
+        tz = nxt[2];
     }
     if (lane == 0) {
         sums[0] = 0.0 + sx;
@@ -170,7 +184,7 @@ int launch_calc_gforce(bx_ctx *c)
 int launch_external_grav(bx_ctx *c)
 {
     const double dx = c->prm.dx, dy = c->prm.dy, dz = c->prm.dz;
-    if (c->grav_flags & BX_GRAV_COM_TREE) {
+    if (!(c->grav_flags & BX_GRAV_COM_SEQUENTIAL)) {
         const int nb = nblocks(c);
         hipLaunchKernelGGL(k_moments_part, dim3(nb), dim3(kGThreads), 0, c->stream, c->n, c->nx, c->ny, c->nz, c->cons,
                            dx, dy, dz, c->grav_sums + 4);
@@ -194,14 +208,17 @@ using namespace bx;
 extern "C" int bx_gravity_config(bx_ctx *c, int32_t flags, double object_mass, double object_radius)
 {
     if (!c) return set_error(BX_ERR_ARG, "bx_gravity_config: null context");
-    if (flags & ~(BX_GRAV_POTENTIAL | BX_GRAV_EXTERNAL | BX_GRAV_COM_TREE))
+    if (flags & ~(BX_GRAV_POTENTIAL | BX_GRAV_EXTERNAL | BX_GRAV_COM_SEQUENTIAL))
         return set_error(BX_ERR_ARG, "bx_gravity_config: unknown flag");
     if (c->prm.precision != BX_PREC_FP64) return set_error(BX_ERR_UNSUPPORTED, "bx_gravity_config: FP64 contexts only");
     if (c->fused_ready || c->adj)
         return set_error(BX_ERR_STATE, "bx_gravity_config: call before the first fused step or adjoint run");
     BX_HIP(hipSetDevice(c->device));
     gravity_free(c);
-    if (!flags) return BX_OK;
+    if (!flags) {
+        c->fused_unsupported = false;  // fused_prepare decides again on the next fused step
+        return BX_OK;
+    }
     BX_HIP(hipMalloc((void **)&c->phi, sizeof(double) * c->n));
     BX_HIP(hipMalloc((void **)&c->gforce, sizeof(double) * 3 * c->n));
     BX_HIP(hipMalloc((void **)&c->grav_sums, sizeof(double) * (4 + 3 * (size_t)nblocks(c))));

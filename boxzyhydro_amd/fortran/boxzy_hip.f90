@@ -167,7 +167,7 @@ module boxzy_hip
      end function
   end interface
 
-  integer, parameter :: BX_GRAV_POTENTIAL = 1, BX_GRAV_EXTERNAL = 2, BX_GRAV_COM_TREE = 4
+  integer, parameter :: BX_GRAV_POTENTIAL = 1, BX_GRAV_EXTERNAL = 2, BX_GRAV_COM_SEQUENTIAL = 4
 
   ! the device counterpart of grid_commons: context plus the step scalars
   type(c_ptr), save :: bxctx = c_null_ptr

@@ -143,7 +143,8 @@ class HydroGrid:
                     object_radius: float = 0.625, initial: bool = True) -> None:
         """Gravity from a given potential ``phi`` (ngrid): BX_GRAV_POTENTIAL
         (calc_gforce), BX_GRAV_EXTERNAL (external_grav's point mass),
-        BX_GRAV_COM_TREE (tree-ordered mass moments).  ``initial`` makes the
+        BX_GRAV_COM_SEQUENTIAL (mass moments in cell order, bit-identical to a
+        one-thread reference; otherwise a tree order).  ``initial`` makes the
         calls of main.f90:146-154 that precede the step loop."""
         self.grav_flags = flags
         nv.check(nv.lib().bx_gravity_config(self._h, flags, object_mass, object_radius), "bx_gravity_config")

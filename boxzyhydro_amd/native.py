@@ -21,7 +21,7 @@ BX_STEP_EXACT = 0
 BX_STEP_FUSED = 1
 BX_GRAV_POTENTIAL = 1
 BX_GRAV_EXTERNAL = 2
-BX_GRAV_COM_TREE = 4
+BX_GRAV_COM_SEQUENTIAL = 4
 
 
 class BxParams(ctypes.Structure):

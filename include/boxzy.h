@@ -285,9 +285,10 @@ int bx_download_muc(bx_ctx *ctx, double *muc_array);
  */
 #define BX_GRAV_POTENTIAL 1 /* calc_gforce from phi (USEPOTENTIAL) */
 #define BX_GRAV_EXTERNAL 2  /* external_grav point mass (EXTERNALGRAV) */
-#define BX_GRAV_COM_TREE 4  /* mass moment by a fixed-order tree sum; without it, a
-                               sequential sum in cell order, bit-identical to the
-                               reference's reduction on one OpenMP thread */
+#define BX_GRAV_COM_SEQUENTIAL 4 /* mass moments summed in cell order, bit-identical to
+                                   the reference's reduction on one OpenMP thread (one
+                                   wave, about 1 s per call at 256^3); without it a
+                                   fixed-order tree sum, equal to rounding */
 /* flags 0 restores the zero-potential force.  object_mass and object_radius
  * are parameters.f90:83-86 (defaults 1.0 and 0.625).  FP64 contexts, before
  * the first fused step.  Sets phi = 0 and gforce = 0 (init_grid.f90:291-299). */

@@ -155,3 +155,21 @@ def test_gpu_fortran_driver_tabulated_eos(tmp_path):
     assert list(dts) == rec["dt"]
     for k in ("cons", "u", "p"):
         assert sha(out[k]) == rec["final_sha256"][k], k
+
+
+@pytest.mark.gpu
+def test_gpu_clearing_gravity_keeps_tabulated_eos_on_exact_steps():
+    """Switching gravity off on a tabulated-EOS context must not re-enable the
+    fixed-gamma fused kernels: the run still matches the reference."""
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+
+    rec = EOS["runs"]["cloud16_h2stat0"]
+    prm, ic = case_inputs(rec)
+    g = HydroGrid(prm, ic, device=0)
+    g.set_gravity(nv.BX_GRAV_POTENTIAL, None, initial=False)
+    g.set_gravity(0)
+    dts = g.run(rec["steps"], mode=nv.BX_STEP_FUSED)
+    assert [float(x) for x in dts] == rec["dt"]
+    assert sha(g.download()["cons"]) == rec["final_sha256"]["cons"]
+    g.close()

@@ -20,7 +20,7 @@ def _flags(rec, tree=False):
     from boxzyhydro_amd import native as nv
 
     f = nv.BX_GRAV_POTENTIAL | (nv.BX_GRAV_EXTERNAL if rec["grav"] == 2 else 0)
-    return f | (nv.BX_GRAV_COM_TREE if tree else 0)
+    return f | (0 if tree else nv.BX_GRAV_COM_SEQUENTIAL)
 
 
 def _grid(rec, tree=False):
@@ -84,8 +84,9 @@ def test_gpu_subroutines_match_reference_snapshots(name):
 
 @pytest.mark.gpu
 def test_gpu_tree_moments_close_to_sequential():
-    """BX_GRAV_COM_TREE sums the mass moments in a fixed tree order instead of
-    cell order: the moments agree to rounding, and so do the runs."""
+    """Without BX_GRAV_COM_SEQUENTIAL the mass moments are summed in a fixed
+    tree order instead of cell order: the moments agree to rounding, and so do
+    the runs."""
     from boxzyhydro_amd import native as nv
 
     rec = GRAV["grav_ext_disk"]
