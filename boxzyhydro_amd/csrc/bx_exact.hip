@@ -408,6 +408,132 @@ __global__ __launch_bounds__(kThreads) void k_ang_to(int64_t n, int nx, int ny, 
     }
 }
 
+// ------------------------------------------ flux: one HLL flux per face
+// The side of cell q in direction d (minus: pm=0, plus: pm=1) exactly as
+// k_flux_states forms it (flux.f90:253-295 slopes, 312-482 states).
+template <int d>
+__device__ __forceinline__ Side<double> cell_side(int64_t q, int pm, int64_t n, int nx, int ny, int nz,
+                                                 int64_t stride, const double *__restrict__ cons,
+                                                 const double *__restrict__ u, const double *__restrict__ p,
+                                                 const double *__restrict__ ad, double dt, double ds, double seps)
+{
+    int ix, iy, iz;
+    coords(q, nx, ny, ix, iy, iz);
+    const bool ghost = boundary_of(ix, iy, iz, nx, ny, nz) > 0;
+    const double rho = cons[q], E = cons[4 * n + q];
+    const double v[3] = {u[q], u[n + q], u[2 * n + q]};
+    const double pc = p[q], gc = ad[q];
+    const double eps = vmax(E - 0.5 * rho * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), seps);
+    const double sound = sqrt(pc * gc / rho);
+    double su[3] = {0.0, 0.0, 0.0}, sp = 0.0, sd = 0.0, se = 0.0, sgm = 0.0;
+    if (!ghost) {
+        const int64_t cm = q - stride, cp = q + stride;
+        for (int j = 0; j < 3; j++) su[j] = slope(u[j * n + cm], v[j], u[j * n + cp]);
+        double em, ep;
+        {
+            const double r = cons[cm];
+            const double a0 = u[cm], a1 = u[n + cm], a2 = u[2 * n + cm];
+            em = vmax(cons[4 * n + cm] - 0.5 * r * (a0 * a0 + a1 * a1 + a2 * a2), seps);
+        }
+        {
+            const double r = cons[cp];
+            const double a0 = u[cp], a1 = u[n + cp], a2 = u[2 * n + cp];
+            ep = vmax(cons[4 * n + cp] - 0.5 * r * (a0 * a0 + a1 * a1 + a2 * a2), seps);
+        }
+        sp = slope(p[cm], pc, p[cp]);
+        sd = slope(cons[cm], rho, cons[cp]);
+        se = slope(em, eps, ep);
+        sgm = slope(ad[cm], gc, ad[cp]);
+    }
+    const double lp = vmax(v[d] + sound, 0.0) * dt / ds;
+    const double lm = vmin(v[d] - sound, 0.0) * dt / ds;
+    const double zc = 0.0 * (dt * 0.5 / ds);  // transverse terms: flux.f90:391-479
+    double mi[7], pl[7];
+    muscl(rho, lm, lp, sd, mi[0], pl[0]);
+    for (int j = 0; j < 3; j++) muscl(v[j], lm, lp, su[j], mi[1 + j], pl[1 + j]);
+    muscl(pc, lm, lp, sp, mi[4], pl[4]);
+    muscl(eps, lm, lp, se, mi[5], pl[5]);
+    muscl(gc, lm, lp, sgm, mi[6], pl[6]);
+    const double *o = pm ? pl : mi;
+    Side<double> sd7;
+    sd7.d = o[0] + zc;
+    sd7.v[0] = o[1] + zc;
+    sd7.v[1] = o[2] + zc;
+    sd7.v[2] = o[3] + zc;
+    sd7.p = o[4] + zc;
+    sd7.e = o[5] + zc;
+    sd7.g = o[6] + zc;
+    return sd7;
+}
+
+// flux.f90:121-190 with each face computed once: F(d, c) is the HLL flux of
+// the face between cell c and c + s_d (get_fluxes/flux_hll, flux.f90:486-590),
+// for the faces some interior cell uses.  The reference forms every face
+// twice, once from each side, from the same inputs and with the same result.
+__global__ __launch_bounds__(kThreads) void k_face_hll(int64_t n, int nx, int ny, int nz, bool flatz,
+                                                      const double *__restrict__ cons, const double *__restrict__ u,
+                                                      const double *__restrict__ p, const double *__restrict__ ad,
+                                                      double *__restrict__ F, double dt, double dx, double dy,
+                                                      double dz, double seps)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int ix, iy, iz;
+    coords(c, nx, ny, ix, iy, iz);
+    const bool in_c = boundary_of(ix, iy, iz, nx, ny, nz) == 0;
+    const int64_t sz = flatz ? 0 : (int64_t)nx * ny;
+    double f[5];
+    // x face c+1/2: needed by c or c+1 when interior
+    if (in_c || (ix < nx && boundary_of(ix + 1, iy, iz, nx, ny, nz) == 0)) {
+        face_flux<double, 0>(cell_side<0>(c, 1, n, nx, ny, nz, 1, cons, u, p, ad, dt, dx, seps),
+                             cell_side<0>(c + 1, 0, n, nx, ny, nz, 1, cons, u, p, ad, dt, dx, seps), f);
+        for (int k = 0; k < 5; k++) F[(int64_t)k * n + c] = f[k];
+    }
+    if (in_c || (iy < ny && boundary_of(ix, iy + 1, iz, nx, ny, nz) == 0)) {
+        face_flux<double, 1>(cell_side<1>(c, 1, n, nx, ny, nz, nx, cons, u, p, ad, dt, dy, seps),
+                             cell_side<1>(c + nx, 0, n, nx, ny, nz, nx, cons, u, p, ad, dt, dy, seps), f);
+        for (int k = 0; k < 5; k++) F[(int64_t)(5 + k) * n + c] = f[k];
+    }
+    if (in_c || (!flatz && iz < nz && boundary_of(ix, iy, iz + 1, nx, ny, nz) == 0)) {
+        face_flux<double, 2>(cell_side<2>(c, 1, n, nx, ny, nz, sz, cons, u, p, ad, dt, dz, seps),
+                             cell_side<2>(c + sz, 0, n, nx, ny, nz, sz, cons, u, p, ad, dt, dz, seps), f);
+        for (int k = 0; k < 5; k++) F[(int64_t)(10 + k) * n + c] = f[k];
+    }
+}
+
+// flux.f90:121-230 on the face fluxes: fluxtmp of interior cells (zero on
+// ghosts, flux.f90:51), the stage update and the floors on every cell.
+__global__ __launch_bounds__(kThreads) void k_face_update(int64_t n, int nx, int ny, int nz, bool flatz,
+                                                         const double *__restrict__ F, double *__restrict__ cons,
+                                                         const double *__restrict__ cons_old, int avg, double dt,
+                                                         double ayz, double axz, double axy, double vol,
+                                                         double srho, double seps)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int ix, iy, iz;
+    coords(c, nx, ny, ix, iy, iz);
+    double ftmp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (boundary_of(ix, iy, iz, nx, ny, nz) == 0) {
+        const int64_t sz = flatz ? 0 : (int64_t)nx * ny;
+        for (int i = 0; i < 5; i++) {
+            const double fx0 = F[(int64_t)i * n + c - 1], fx1 = F[(int64_t)i * n + c];
+            const double fy0 = F[(int64_t)(5 + i) * n + c - nx], fy1 = F[(int64_t)(5 + i) * n + c];
+            const double fz0 = F[(int64_t)(10 + i) * n + c - sz], fz1 = F[(int64_t)(10 + i) * n + c];
+            ftmp[i] = -((ayz * (fx1 - fx0) + axz * (fy1 - fy0) + axy * (fz1 - fz0)) * dt / vol);
+        }
+    }
+    double U[5];
+    for (int i = 0; i < 5; i++) {
+        const double ft = ftmp[i];
+        const double cu = cons[(int64_t)i * n + c];
+        U[i] = avg == 0 ? cu + ft : 0.5 * (cons_old[(int64_t)i * n + c] + cu + ft);
+    }
+    U[0] = vmax(U[0], srho);
+    if (U[4] < 0.0) U[4] = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0] + seps;
+    for (int i = 0; i < 5; i++) cons[(int64_t)i * n + c] = U[i];
+}
+
 // ------------------------------------------------- flux: faces and update
 // flux.f90:121-230: six HLL faces and fluxtmp on interior cells (fluxtmp=0 on
 // ghosts, flux.f90:51), then the stage update and floors on every cell.
@@ -586,15 +712,22 @@ int launch_flux(bx_ctx *c, int avg, double dt)
                            avg, dx, dy, c->prm.yoffset);
         BX_HIP(hipGetLastError());
     }
-    auto states = ang ? k_flux_states<true> : k_flux_states<false>;
-    hipLaunchKernelGGL(states, grid, dim3(kThreads), 0, c->stream, c->n, c->nx, c->ny, c->nz, c->flatz, c->cons, c->u,
-                       c->p, c->adindx, c->states, dt, dx, dy, dz, c->prm.small_eps);
-    BX_HIP(hipGetLastError());
     const double areaxy = dy * dx, areayz = dy * dz, areaxz = dx * dz, vol = dx * dy * dz;
-    auto faces = ang ? k_flux_faces<true> : k_flux_faces<false>;
-    hipLaunchKernelGGL(faces, grid, dim3(kThreads), 0, c->stream, c->n, c->nx, c->ny, c->nz, c->flatz, c->states,
-                       c->cons, c->cons_old, avg, dt, areayz, areaxz, areaxy, vol, c->prm.small_rho,
-                       c->prm.small_eps, dx, dy, c->prm.yoffset);
+    if (ang) {  // interface states of every cell, then the faces per cell (the pressure projection is per cell)
+        hipLaunchKernelGGL(k_flux_states<true>, grid, dim3(kThreads), 0, c->stream, c->n, c->nx, c->ny, c->nz,
+                           c->flatz, c->cons, c->u, c->p, c->adindx, c->states, dt, dx, dy, dz, c->prm.small_eps);
+        BX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_flux_faces<true>, grid, dim3(kThreads), 0, c->stream, c->n, c->nx, c->ny, c->nz,
+                           c->flatz, c->states, c->cons, c->cons_old, avg, dt, areayz, areaxz, areaxy, vol,
+                           c->prm.small_rho, c->prm.small_eps, dx, dy, c->prm.yoffset);
+    } else {  // each face's flux once, then the update
+        hipLaunchKernelGGL(k_face_hll, grid, dim3(kThreads), 0, c->stream, c->n, c->nx, c->ny, c->nz, c->flatz,
+                           c->cons, c->u, c->p, c->adindx, c->states, dt, dx, dy, dz, c->prm.small_eps);
+        BX_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_face_update, grid, dim3(kThreads), 0, c->stream, c->n, c->nx, c->ny, c->nz, c->flatz,
+                           c->states, c->cons, c->cons_old, avg, dt, areayz, areaxz, areaxy, vol, c->prm.small_rho,
+                           c->prm.small_eps);
+    }
     BX_HIP(hipGetLastError());
     return launch_ghosts(c, c->cons);
 }

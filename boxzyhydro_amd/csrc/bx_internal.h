@@ -54,7 +54,7 @@ struct bx_ctx {
     double *p;
     double *adindx;
     double *qq;        // 3 planes
-    double *states;    // 42 planes: per direction 7 minus + 7 plus interface states
+    double *states;    // exact-flux intermediate, state_planes(c) planes (face fluxes or interface states)
 
     // ghost table: ghost cell id, donor (>=0: cell id; <0: -(scratch slot+1)),
     // sign bits (bit k flips momentum k); scratch holds pre-fill values of
@@ -114,9 +114,10 @@ int hip_check(hipError_t e, const char *what);
         if (_e != hipSuccess) return bx::hip_check(_e, #call);   \
     } while (0)
 
-// interface-state planes of the exact flux: per direction 7 minus + 7 plus
-// (rho, u, v, w, p, eps, gamma), plus r- and angular momentum with flux_ang
-inline int state_planes(const bx_ctx *c) { return c->prm.fluxangmom ? 54 : 42; }
+// planes of the exact flux's intermediate: the face fluxes, 5 per direction;
+// with flux_ang the interface states, per direction 9 minus + 9 plus (rho, u,
+// v, w, p, eps, gamma, r- and angular momentum)
+inline int state_planes(const bx_ctx *c) { return c->prm.fluxangmom ? 54 : 15; }
 
 inline unsigned int blocks_for(int64_t n, int threads)
 {
