@@ -87,7 +87,8 @@ __global__ __launch_bounds__(64) void k_moments_seq(int64_t n, int nx, int ny, i
 }
 
 // Fixed-order tree: block partials (pairwise in shared memory), then one block
-// folds the partials the same way.  Deterministic, not the sequential order.
+// folds the partials (strided per thread, then pairwise).  Deterministic, not
+// the sequential order.
 __device__ __forceinline__ void block_tree3(double v[3], double (*sh)[kGThreads])
 {
     for (int k = 0; k < 3; k++) sh[k][threadIdx.x] = v[k];
@@ -117,16 +118,11 @@ __global__ __launch_bounds__(kGThreads) void k_moments_final(int nb, const doubl
                                                             double *__restrict__ sums)
 {
     __shared__ double sh[3][kGThreads];
+    // thread t folds partials t, t+256, ... in order, then one tree over the threads
     double v[3] = {0.0, 0.0, 0.0};
-    for (int b0 = 0; b0 < nb; b0 += kGThreads) {  // strided chunks, each folded by the tree
-        double w[3] = {0.0, 0.0, 0.0};
-        const int b = b0 + threadIdx.x;
-        if (b < nb)
-            for (int k = 0; k < 3; k++) w[k] = part[(int64_t)k * nb + b];
-        block_tree3(w, sh);
-        for (int k = 0; k < 3; k++) v[k] = v[k] + w[k];
-        __syncthreads();
-    }
+    for (int b = threadIdx.x; b < nb; b += kGThreads)
+        for (int k = 0; k < 3; k++) v[k] = v[k] + part[(int64_t)k * nb + b];
+    block_tree3(v, sh);
     if (threadIdx.x == 0)
         for (int k = 0; k < 3; k++) sums[k] = 0.0 + v[k];
 }
