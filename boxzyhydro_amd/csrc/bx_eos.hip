@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <new>
 #include <thread>
 #include <vector>
 
@@ -408,13 +409,18 @@ struct bx_eos {
 
 namespace bx {
 
-void eos_free(bx_ctx *c)
+static void eos_release(bx_eos *q)
 {
-    if (!c || !c->eos) return;
-    bx_eos *q = c->eos;
+    if (!q) return;
     for (void *d : {(void *)q->tab, (void *)q->rows, (void *)q->muc_plane})
         if (d) (void)hipFree(d);
     delete q;
+}
+
+void eos_free(bx_ctx *c)
+{
+    if (!c) return;
+    eos_release(c->eos);
     c->eos = nullptr;
 }
 
@@ -508,7 +514,6 @@ extern "C" int bx_eos_init(bx_ctx *c, const bx_eos_params *e)
     Tables t;
     int rc = build_tables(&c->prm, e, t);
     if (rc) return rc;
-    eos_free(c);
     const size_t N = (size_t)t.neos * t.nrho;
     auto entry = [&](size_t k, int it) {
         TabEntry q{t.eng2[k], t.gam2[k], t.muc2[k], t.tk2[k], 0.0, 0.0, 0.0, 0.0};
@@ -534,17 +539,25 @@ extern "C" int bx_eos_init(bx_ctx *c, const bx_eos_params *e)
         rows[ir] = RhoRow{r0, r0 == r1 ? std::nan("") : 1.0 / (r1 - r0), std::log10(t.eng2[o]), t.deng[ir],
                           t.eng2[o], t.eng2[o + t.neos - 1], 0.0, 0.0};
     }
-    bx_eos *q = new bx_eos();
-    c->eos = q;
-    BX_HIP(hipMalloc(&q->tab, N * sizeof(TabPair)));
-    BX_HIP(hipMalloc(&q->rows, t.nrho * sizeof(RhoRow)));
-    BX_HIP(hipMalloc(&q->muc_plane, c->n * sizeof(double)));
-    BX_HIP(hipMemcpy(q->tab, pk.data(), N * sizeof(TabPair), hipMemcpyHostToDevice));
-    BX_HIP(hipMemcpy(q->rows, rows.data(), t.nrho * sizeof(RhoRow), hipMemcpyHostToDevice));
+    // built aside and attached only when complete, so a failed allocation leaves
+    // the context on its previous EOS
+    bx_eos *q = new (std::nothrow) bx_eos();
+    if (!q) return set_error(BX_ERR_ARG, "bx_eos_init: out of host memory");
+    hipError_t err = hipMalloc(&q->tab, N * sizeof(TabPair));
+    if (err == hipSuccess) err = hipMalloc(&q->rows, t.nrho * sizeof(RhoRow));
+    if (err == hipSuccess) err = hipMalloc(&q->muc_plane, c->n * sizeof(double));
+    if (err == hipSuccess) err = hipMemcpy(q->tab, pk.data(), N * sizeof(TabPair), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(q->rows, rows.data(), t.nrho * sizeof(RhoRow), hipMemcpyHostToDevice);
     // init_grid.f90:294: muc_array starts at zero
-    BX_HIP(hipMemset(q->muc_plane, 0, c->n * sizeof(double)));
+    if (err == hipSuccess) err = hipMemset(q->muc_plane, 0, c->n * sizeof(double));
+    if (err != hipSuccess) {
+        eos_release(q);
+        return hip_check(err, "bx_eos_init: table upload");
+    }
     const Units scl = get_units();
     q->dev = EosDev{t.neos, t.nrho, t.log_rho_low, t.drho, scl.density, e->tk_bgrnd, q->tab, q->rows};
+    eos_free(c);
+    c->eos = q;
     // the fused kernels assume a fixed gamma: bx_step's fused mode runs exact steps
     c->fused_unsupported = true;
     return BX_OK;
