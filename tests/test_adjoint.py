@@ -144,3 +144,47 @@ def test_adjoint_deterministic_and_checkpoint_independent():
     assert torch.equal(g1[0], g3[0]) and torch.equal(g1[1], g3[1])
     # ghost cells carry no gradient (their values are overwritten by the fill)
     assert float(g1[0][~interior].abs().max()) == 0.0
+
+
+@pytest.mark.gpu
+def test_adjoint_full_size_matches_central_differences():
+    """The adjoint workload at its full size: 128^3 box, 200 steps, checkpoints
+    every 20, loss 0.5|rho_T - rho_target|^2 with the target from a second
+    seed.  64 seeded interior cells, rho and p each, against central
+    differences with eps = 1e-6: |fd - adjoint| <= 1e-6 max|g|.  Over 200
+    steps a +-eps stencil can straddle a branch switch of the limiter or the
+    HLL speed bounds somewhere in the cell's domain of dependence: such an
+    entry passes only when its one-sided differences disagree (a kink inside
+    the stencil) and a smaller eps agrees -- at most 4 of 128."""
+    import torch
+
+    prm, st, rho0, p0, tgt, interior = _setup(n=128, nsteps=200, every=20)
+    g_rho, g_p, _ = _grad(st, rho0, p0, tgt, interior)
+    rng = np.random.Generator(np.random.PCG64(11))
+    cells = rng.choice(np.flatnonzero(interior.cpu().numpy()), size=64, replace=False)
+    scale = max(float(g_rho.abs().max()), float(g_p.abs().max()))
+    tol = 1e-6 * scale
+    zero = torch.zeros_like(rho0)
+    kinks, errs = [], []
+    print(f"128^3 x 200 steps, max|g| {scale:.4e}", flush=True)
+    for c in cells:
+        for which, g in (("rho", g_rho), ("p", g_p)):
+            e = torch.zeros_like(rho0)
+            e[c] = 1.0
+            d_rho, d_p = (e, zero) if which == "rho" else (zero, e)
+            ad = float(g[c])
+            fd, fwd, bwd = _central(st, rho0, p0, d_rho, d_p, EPS, tgt, interior)
+            err = abs(fd - ad)
+            if err <= tol:
+                errs.append(err / scale)
+                continue
+            print(f"cell {c} {which}: adjoint {ad:+.12e} fd {fd:+.12e} err/max|g| {err / scale:.2e} "
+                  f"one-sided gap/max|g| {abs(fwd - bwd) / scale:.2e}", flush=True)
+            assert abs(fwd - bwd) > tol, (c, which, fd, fwd, bwd, ad)
+            refined = min(abs(_central(st, rho0, p0, d_rho, d_p, h, tgt, interior)[0] - ad) for h in (1e-7, 1e-8))
+            print(f"   refined err/max|g| {refined / scale:.2e}", flush=True)
+            assert refined <= tol, (c, which)
+            kinks.append((int(c), which))
+    print(f"smooth entries: {len(errs)}, worst err/max|g| {max(errs):.2e}, median {np.median(errs):.2e}; "
+          f"kinks {kinks}", flush=True)
+    assert len(kinks) <= 4
