@@ -7,6 +7,7 @@
   step modes.
 * 512^3 box (137 M cells with ghosts, 64-bit plane offsets beyond 2^30
   elements): the fused step equals the exact call sequence bit for bit.
+* 384^3 box, 100 steps of the mixed form against FP64: rho and p L1 <= 1e-6.
 """
 import json
 import os
@@ -81,3 +82,34 @@ def test_gpu_512_cube_fused_equals_exact():
     assert list(df) == list(de)
     assert np.array_equal(cf.view(np.uint64), ce.view(np.uint64))
     assert np.isfinite(cf).all()
+
+
+@pytest.mark.gpu
+def test_gpu_mixed_384_100_steps_error_budget():
+    """The mixed-precision workload at its full size: the box generator at
+    384^3, 100 steps of the mixed form (FP32 storage and reconstruction, FP64
+    update and pressure recovery) against the FP64 step: relative L1 error of
+    rho and p <= 1e-6.  (Its speed is measured by bench.py.)"""
+    from boxzyhydro_amd import ics
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+    from boxzyhydro_amd.params import box_params
+
+    prm = box_params(384)
+    print("box 384^3: generating", flush=True)
+    ic = ics.box_ic(prm, seed=0)
+    out = {}
+    for name, prec in (("fp64", nv.BX_PREC_FP64), ("mixed", nv.BX_PREC_MIXED)):
+        print(f"box 384^3 x 100 steps, {name}", flush=True)
+        g = HydroGrid(prm, ic, device=0, precision=prec)
+        g.run(100, mode=nv.BX_STEP_FUSED)
+        out[name] = (g.download(), g.time)
+        g.close()
+    interior = ics.boundary_flags(prm).reshape(-1) == 0
+    a, b = out["fp64"][0], out["mixed"][0]
+    for name, x, y in (("rho", a["cons"][:, 0], b["cons"][:, 0]), ("p", a["p"], b["p"])):
+        x, y = x[interior], y[interior]
+        l1 = np.abs(x - y).sum() / np.abs(x).sum()
+        print(f"{name}: L1 {l1:.3e} Linf {np.abs(x - y).max() / np.abs(x).max():.3e}", flush=True)
+        assert l1 <= 1e-6, (name, l1)
+    assert abs(out["fp64"][1] - out["mixed"][1]) < 1e-5 * out["fp64"][1]
