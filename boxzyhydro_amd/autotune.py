@@ -54,3 +54,15 @@ def tune_batch(batch, candidates: Iterable[Tuple[int, int]] = BATCH_CANDIDATES, 
     best = min(results, key=lambda r: r[1])[0]
     batch.tune(*best)
     return best, results
+
+
+# the batched-grid kernel holds about 210 registers per lane: at most 8 waves per CU
+BOX_CANDIDATES = [(8, 1), (4, 1), (4, 2)]
+
+
+def tune_boxes(batch, candidates: Iterable[Tuple[int, int]] = BOX_CANDIDATES, reps: int = 2,
+               graph: bool = True) -> Tuple[Tuple[int, int], List[Tuple[Tuple[int, int], float]]]:
+    """Work split of the batched-grid launch (waves per workgroup, workgroups
+    per compute unit), timed like tune_batch.  Results do not depend on the
+    choice."""
+    return tune_batch(batch, candidates, reps, graph)

@@ -15,6 +15,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libboxzy_hip.so")
 
 BX_OK = 0
+BX_ERR_ARG = -1
+BX_ERR_HIP = -2
+BX_ERR_STATE = -3
+BX_ERR_UNSUPPORTED = -4
 BX_PREC_FP64 = 0
 BX_PREC_MIXED = 1
 BX_STEP_EXACT = 0
@@ -101,6 +105,16 @@ _SIGS = {
     "bx_batch_sync": [_VP, _DP],
     "bx_batch_result": [_VP, _DP, _DP, _DP, _DP, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_uint32)],
     "bx_batch_dts": [_VP, _DP],
+    "bx_boxes_create": [_I32, ctypes.POINTER(BxParams), _DP, _DP, _DP, _DP, _I32, ctypes.POINTER(_VP)],
+    "bx_boxes_destroy": [_VP],
+    "bx_boxes_cells": [_VP, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
+    "bx_boxes_order": [_VP, ctypes.POINTER(_I32)],
+    "bx_boxes_tune": [_VP, _I32, _I32],
+    "bx_boxes_log_dt": [_VP, _I32],
+    "bx_boxes_run": [_VP, _I32, _I32],
+    "bx_boxes_sync": [_VP, _DP],
+    "bx_boxes_result": [_VP, _DP, _DP, _DP, _DP, ctypes.POINTER(_I32), ctypes.POINTER(ctypes.c_uint32)],
+    "bx_boxes_dts": [_VP, _DP],
     "bx_adjoint_forward": [_VP, _I32, _I32, _DP, _DP],
     "bx_adjoint_backward": [_VP, _VP, _VP],
     "bx_adjoint_tune": [_VP, _I32, _I32],

@@ -179,6 +179,59 @@ int bx_batch_result(bx_batch *b, double *cons_rows, double *u_rows, double *p_ro
 int bx_batch_dts(bx_batch *b, double *dts);
 
 /* ------------------------------------------------------------------------
+ * Batched grids.  Many independent grids of uneven size (3-D boxes, or flat-z
+ * slabs with nz < 6) in one persistent launch: one workgroup runs one grid
+ * through the call sequence of main.f90:211-555 (the one-kernel-per-
+ * subroutine path's per-cell code, subroutine by subroutine), taking grids
+ * from a queue ordered by estimated cost, largest first.  No reference
+ * counterpart to cite: the reference runs one grid per process, and each grid
+ * here returns what that process returns, bit for bit.  Every grid has its
+ * own bx_params (size, spacings, walls, gammafix, cfl, small_rho/eps,
+ * vlimit, abundances) and endtime; the EOS is the fixed-gamma one
+ * (H2STAT=-1), the flux the plain one (fluxangmom=0), the force that of a
+ * zero potential, and the precision FP64.
+ *
+ * Host arrays are the grids' Fortran-layout arrays concatenated in order:
+ * cons(5,.), u(3,.), p(.) over sum(nx*ny*nz) cells.  Suited to grids of up to
+ * about 10^5 cells each (a grid is one workgroup's work). */
+typedef struct bx_boxes bx_boxes;
+
+/* Upload the grids' set-ups; grid i runs until time >= endtime[i]. */
+int bx_boxes_create(int32_t nbox, const bx_params *prm, const double *endtime, const double *cons, const double *u,
+                    const double *p, int32_t device, bx_boxes **out);
+int bx_boxes_destroy(bx_boxes *b);
+
+/* Cells of all grids (ghosts included), and their interior cells. */
+int bx_boxes_cells(const bx_boxes *b, int64_t *cells, int64_t *interior_cells);
+
+/* Queue order (grid indices, largest estimated cost first). */
+int bx_boxes_order(const bx_boxes *b, int32_t *order);
+
+/* Work split: waves per workgroup (4 or 8) and workgroups per compute unit
+ * (at most 8 waves per CU).  Used by the autotuner; results do not change. */
+int bx_boxes_tune(bx_boxes *b, int32_t waves_per_block, int32_t blocks_per_cu);
+
+/* Keep the first maxlog dt values of every grid (0: off). */
+int bx_boxes_log_dt(bx_boxes *b, int32_t maxlog);
+
+/* Run every grid from its set-up to its endtime (at most maxsteps steps)
+ * with one persistent launch; use_graph replays a captured hipGraph of the
+ * launch.  Asynchronous; a later run starts again from the set-ups. */
+int bx_boxes_run(bx_boxes *b, int32_t maxsteps, int32_t use_graph);
+
+/* Wait for the run; *ms receives its duration on the device (HIP events). */
+int bx_boxes_sync(bx_boxes *b, double *ms);
+
+/* Final cons(5,.), u(3,.), p(.) (any may be NULL), time, steps and status
+ * bits per grid (4: a NaN, zero or infinite dt stopped the grid before that
+ * step). */
+int bx_boxes_result(bx_boxes *b, double *cons, double *u, double *p, double *time, int32_t *steps,
+                    uint32_t *status);
+
+/* dt log, nbox x maxlog (entries past a grid's step count are unset). */
+int bx_boxes_dts(bx_boxes *b, double *dts);
+
+/* ------------------------------------------------------------------------
  * Adjoint of the FP64 step with a frozen dt sequence (no reference
  * counterpart: the reference has no derivatives).  The differentiated map is
  * the step of main.f90:211-555 on a regular state (ghost cells copies of their
