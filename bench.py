@@ -160,6 +160,70 @@ def batched(n_tubes: int, reps: int):
             "status_bits": int(np.bitwise_or.reduce([r["status"] for r in out]))}
 
 
+def box_workload(n_boxes: int, seed: int = 0):
+    """Uneven 3-D boxes: interior cells per axis log-uniform in [8, 32] (each
+    axis drawn separately), walls on a random subset of sides, seeded smoothed
+    rho/p with rms Mach 0.5, gamma 5/3, CFL 0.5, run to t_end = 2."""
+    import numpy as np
+
+    from boxzyhydro_amd import ics
+    from boxzyhydro_amd.params import HydroParams
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n_boxes):
+        nx, ny, nz = (int(round(v)) + 4 for v in np.exp(rng.uniform(np.log(8), np.log(32), 3)))
+        walls = [bool(w) for w in rng.integers(0, 2, 6)]
+        prm = HydroParams(nx=nx, ny=ny, nz=nz, dx=0.25, dy=0.25, dz=0.25, gammafix=5.0 / 3.0, cfl=0.5,
+                          endtime=2.0, no_outflow_xl=walls[0], no_outflow_xr=walls[1], no_outflow_yl=walls[2],
+                          no_outflow_yr=walls[3], no_outflow_zl=walls[4], no_outflow_zr=walls[5])
+        out.append((prm, ics.box_ic(prm, seed=seed * 100000 + i, sigma_cells=2.0)))
+    return out
+
+
+def boxes_cost(n_boxes: int, reps: int, one_at_a_time: bool = True):
+    """The uneven-box workload in one persistent launch replayed from a
+    captured hipGraph (work split picked by the autotuner), against the same
+    boxes run one after another through the one-kernel-per-subroutine path."""
+    import numpy as np
+
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.autotune import tune_boxes
+    from boxzyhydro_amd.boxes import GridBatch
+    from boxzyhydro_amd.hydro import HydroGrid
+
+    grids = box_workload(n_boxes)
+    b = GridBatch(grids)
+    best, res = tune_boxes(b, reps=1)
+    b.run(maxsteps=100000, graph=True)
+    b.sync()
+    ms = min([(b.run(maxsteps=100000, graph=True), b.sync())[1] for _ in range(reps)])
+    out = b.result()
+    steps = np.array([r["steps"] for r in out])
+    interior = np.array([prm.interior_cells for prm, _ in grids])
+    updates = float(np.sum(steps * interior))
+    b.close()
+    seq_s = 0.0
+    for prm, ic in (grids if one_at_a_time else []):
+        g = HydroGrid(prm, ic, device=0)
+        g.sync()
+        t0 = time.perf_counter()
+        g.run(100000, mode=nv.BX_STEP_EXACT, record_dt=False)
+        g.sync()
+        seq_s += time.perf_counter() - t0
+        g.close()
+    return {"workload": f"{n_boxes} uneven 3-D boxes, interior per axis log-uniform in [8,32], random walls, "
+                        f"gamma 5/3, CFL 0.5, t_end 2, FP64, one persistent launch replayed from a hipGraph",
+            "ms": ms, "cells_interior": int(interior.sum()), "steps_min": int(steps.min()),
+            "steps_max": int(steps.max()), "cell_updates_per_s": updates / (ms * 1e-3),
+            "one_grid_at_a_time_s": seq_s if one_at_a_time else None,
+            "one_grid_at_a_time_cell_updates_per_s": updates / seq_s if one_at_a_time else None,
+            "speedup_vs_one_at_a_time": seq_s / (ms * 1e-3) if one_at_a_time else None,
+            "tile": {"waves_per_block": best[0], "blocks_per_cu": best[1]},
+            "tile_ms": {f"{c[0]}x{c[1]}": t * 1e3 for c, t in res},
+            "status_bits": int(np.bitwise_or.reduce([r["status"] for r in out]))}
+
+
 def adjoint_cost(n: int, nsteps: int, every: int):
     """Backward (recompute from checkpoints + reverse sweep) against the plain
     forward of the same frozen-dt map, on the adjoint workload."""
@@ -336,6 +400,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision line")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched shock tubes")
+    ap.add_argument("--no-boxes", action="store_true", help="skip the batched uneven boxes")
+    ap.add_argument("--boxes-only", action="store_true", help="run only the batched uneven boxes")
+    ap.add_argument("--no-boxes-seq", action="store_true",
+                    help="batched boxes without the one-grid-at-a-time comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost")
     ap.add_argument("--no-eos", action="store_true", help="skip the tabulated-EOS line")
     ap.add_argument("--no-options", action="store_true", help="skip the flux_ang / gravity line")
@@ -376,6 +444,9 @@ def main() -> int:
         return 0
     if args.options_only:
         print(json.dumps({"options": options_cost(args.size, 3)}))
+        return 0
+    if args.boxes_only:
+        print(json.dumps({"boxes": boxes_cost(512, 3, not args.no_boxes_seq)}))
         return 0
     prm = box_params(args.size)
     log(f"generating box {args.size}^3 ...")
@@ -442,6 +513,9 @@ def main() -> int:
     if not args.no_batch:
         log("batched shock tubes ...")
         out["batched"] = batched(512, 3)
+    if not args.no_boxes:
+        log("batched uneven boxes ...")
+        out["boxes"] = boxes_cost(512, 3, not args.no_boxes_seq)
     if not args.no_eos:
         log("tabulated EOS ...")
         out["eos"] = eos_cost(args.size, 5, 20)
