@@ -35,9 +35,11 @@ enum : unsigned int {
     BOX_BADDT = 4u,  // a non-finite or non-positive dt: the grid was stopped before that step
 };
 
-// Per-grid region, planes of n doubles: cons 5 | cons_old 5 | u 3 | p | adindx
-// | face fluxes 15, then the ghost-donor snapshot (5 planes of nscratch).
+// Per-grid region, planes of n doubles: cons 5 | u 3 | p | adindx | cons_old 5
+// | face fluxes 15, then the ghost-donor snapshot (5 planes of nscratch).  The
+// first 9 planes are the result (cons, u, p).
 constexpr int kRegionPlanes = 30;
+constexpr int kResultPlanes = 9;
 // Set-up, planes of n doubles: cons 5 | u 3 | p
 constexpr int kSetupPlanes = 9;
 
@@ -125,10 +127,10 @@ __global__ __launch_bounds__(64 * W) void k_boxes(BoxArgs a)
         const Box b = a.boxes[bi];
         const int64_t n = b.n;
         double *cons = a.reg + b.reg_off;
-        double *old = cons + 5 * n;
-        double *u = cons + 10 * n;
-        double *p = cons + 13 * n;
-        double *ad = cons + 14 * n;
+        double *u = cons + 5 * n;
+        double *p = cons + 8 * n;
+        double *ad = cons + 9 * n;
+        double *old = cons + 10 * n;
         double *F = cons + 15 * n;
         double *scratch = cons + (int64_t)kRegionPlanes * n;
         const double *ic = a.ic + b.ic_off;
@@ -638,18 +640,26 @@ int bx_boxes_result(bx_boxes *b, double *cons, double *u, double *p, double *tim
     BX_HIP(hipStreamSynchronize(b->stream));
     const int nb = b->nbox;
     if (cons || u || p) {
-        std::vector<double> reg((size_t)b->reg_total);
-        BX_HIP(hipMemcpy(reg.data(), b->reg, sizeof(double) * reg.size(), hipMemcpyDeviceToHost));
+        // the result planes of each grid (cons, u, p), gathered into one host buffer
+        std::vector<double> res((size_t)(b->cells * bx::kResultPlanes));
         int64_t cell0 = 0;
         for (int i = 0; i < nb; i++) {
             const int64_t n = b->hbox[i].n;
-            const double *r = reg.data() + b->hbox[i].reg_off;
+            BX_HIP(hipMemcpyAsync(res.data() + cell0 * bx::kResultPlanes, b->reg + b->hbox[i].reg_off,
+                                  sizeof(double) * bx::kResultPlanes * n, hipMemcpyDeviceToHost, b->stream));
+            cell0 += n;
+        }
+        BX_HIP(hipStreamSynchronize(b->stream));
+        cell0 = 0;
+        for (int i = 0; i < nb; i++) {
+            const int64_t n = b->hbox[i].n;
+            const double *r = res.data() + cell0 * bx::kResultPlanes;
             for (int64_t c = 0; c < n; c++) {
                 if (cons)
                     for (int f = 0; f < 5; f++) cons[(cell0 + c) * 5 + f] = r[f * n + c];
                 if (u)
-                    for (int f = 0; f < 3; f++) u[(cell0 + c) * 3 + f] = r[(10 + f) * n + c];
-                if (p) p[cell0 + c] = r[13 * n + c];
+                    for (int f = 0; f < 3; f++) u[(cell0 + c) * 3 + f] = r[(5 + f) * n + c];
+                if (p) p[cell0 + c] = r[8 * n + c];
             }
             cell0 += n;
         }
