@@ -312,9 +312,10 @@ int bx_destroy(bx_ctx *c)
     adjoint_free(c);
     eos_free(c);
     gravity_free(c);
+    drop_step_graphs(c);
     void *dev[] = {c->cons, c->cons_old, c->u, c->p, c->adindx, c->qq, c->states, c->ghost_dst, c->ghost_src,
                    c->ghost_sign, c->ghost_scratch_src, c->ghost_scratch, c->ustar, c->pstar, c->unext, c->dsc,
-                   c->cons32, c->ustar32, c->unext32, c->pstar32, c->eflux};
+                   c->cons32, c->ustar32, c->unext32, c->pstar32, c->eflux, c->chunk0, c->chunk_dts};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (c->hsc) (void)hipHostFree(c->hsc);
@@ -543,6 +544,13 @@ int bx_fallback_count(const bx_ctx *c, int64_t *count)
     return BX_OK;
 }
 
+int bx_graph_step_count(const bx_ctx *c, int64_t *count)
+{
+    if (!c || !count) return set_error(BX_ERR_ARG, "bx_graph_step_count: null argument");
+    *count = c->graph_steps;
+    return BX_OK;
+}
+
 }  // extern "C"
 
 namespace bx {
@@ -635,12 +643,77 @@ static int read_scalars(bx_ctx *c)
     return BX_OK;
 }
 
-// Fused stepping (see include/boxzy.h): returns after `nsteps` steps or endtime.
+// Steps (an even number) that can be replayed from the current state with
+// none of them reaching endtime: with dtold > 0 each dt is at most 1.1x the
+// previous one (main.f90:271-273), so the times stay below these bounds (taken
+// with a margin far above the rounding of the bound itself).  Then every step
+// of the chunk is taken, none is clipped, and the loop condition holds before
+// each.
+static int chunk_steps(double time, double dtold, double endtime, int32_t step, int32_t left)
+{
+    if (!(dtold > 0.0) || !std::isfinite(dtold) || !std::isfinite(time) || step < 1) return 0;
+    const double m = 1.0 + 1e-9;
+    double t = time, d = dtold;
+    int k = 0;
+    while (k < kChunkSteps && k < left && step + k < 1000000) {
+        d = d * 1.1 * m;
+        t = (t + d) * m;
+        if (!(t < endtime)) break;
+        k++;
+    }
+    return k & ~1;
+}
+
+// K steps replayed from the captured pair (BX_STEP_GRAPH).  The state at the
+// chunk's start is kept; if any step met an irregular branch the chunk is
+// undone (*taken = 0) and the caller takes those steps one at a time.
+static int graph_chunk(bx_ctx *c, int K, double endtime, double *time, double *dtold, int32_t *step, double *dts,
+                       int32_t *taken)
+{
+    *taken = 0;
+    const size_t bytes = (c->state_in_32 ? sizeof(float) : sizeof(double)) * 5 * (size_t)c->n;
+    void *cur = c->state_in_32 ? (void *)c->cons32 : (void *)c->cons;
+    if (!c->chunk0) BX_HIP(hipMalloc(&c->chunk0, sizeof(double) * 5 * (size_t)c->n));
+    BX_HIP(hipMemcpyAsync(c->chunk0, cur, bytes, hipMemcpyDeviceToDevice, c->stream));
+    const unsigned long long rate0 = c->hsc->rate_bits[0];  // Courant rate of the current state
+    DevScalars *h = c->hsc;
+    h->time = *time;
+    h->dtold = *dtold;
+    h->step = *step;
+    h->endtime = endtime;
+    h->flags = 0;
+    h->logk = 0;
+    BX_HIP(hipMemcpyAsync(c->dsc, h, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream));
+    if (int rc = replay_step_pairs(c, K / 2)) return rc;
+    double log[kChunkSteps];
+    BX_HIP(hipMemcpyAsync(log, c->chunk_dts, sizeof(double) * K, hipMemcpyDeviceToHost, c->stream));
+    if (int rc = read_scalars(c)) return rc;
+    if ((h->flags & FLAG_IRREGULAR) || h->step != *step + K) {
+        BX_HIP(hipMemcpyAsync(cur, c->chunk0, bytes, hipMemcpyDeviceToDevice, c->stream));
+        h->rate_bits[0] = rate0;
+        return BX_OK;
+    }
+    *time = h->time;
+    *dtold = h->dtold;
+    *step = h->step;
+    if (dts)
+        for (int i = 0; i < K; i++) dts[i] = log[i];
+    c->aux_stale = true;
+    c->old_stale = true;
+    c->graph_steps += K;
+    *taken = K;
+    return BX_OK;
+}
+
+// Fused stepping (see include/boxzy.h): returns after `nsteps` steps or
+// endtime.  With `graph`, runs of steps that cannot reach endtime are replayed
+// from a captured pair of steps, with dt formed on the device.
 static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, double *dtold, int32_t *step,
-                      double *dts, int32_t *taken)
+                      double *dts, int32_t *taken, bool graph)
 {
     int rc;
     int32_t k = 0;
+    int32_t single = 0;  // steps to take one at a time (after an undone chunk or an irregular step)
     // While fused_valid holds, hsc->rate_bits[0] is the Courant rate of the
     // current state (from the regularity check or the last fused step).
     while (k < nsteps && *time < endtime && *step < 1000000) {
@@ -670,6 +743,19 @@ static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, d
                 c->state_in_32 = true;
             }
         }
+        if (graph && single == 0 && !c->prof_on) {
+            const int K = chunk_steps(*time, *dtold, endtime, *step, nsteps - k);
+            if (K >= 2) {
+                int32_t got = 0;
+                if ((rc = graph_chunk(c, K, endtime, time, dtold, step, dts ? dts + k : nullptr, &got))) return rc;
+                if (got) {
+                    k += got;
+                    continue;
+                }
+                single = K;
+            }
+        }
+        if (single > 0) single--;
         const double rate = rate_from_bits(c->hsc->rate_bits[0]);
         const double time0 = *time, dtold0 = *dtold;
         const int32_t step0 = *step;
@@ -679,7 +765,9 @@ static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, d
         if ((rc = read_scalars(c))) return rc;
         if (c->hsc->flags & FLAG_IRREGULAR) {
             // an irregular branch: redo this step with the exact sequence from U^n
+            // (replays resume after a full chunk's worth of regular steps)
             c->fallbacks++;
+            if (graph) single = kChunkSteps;
             *time = time0;
             *dtold = dtold0;
             *step = step0;
@@ -723,10 +811,13 @@ extern "C" int bx_step(bx_ctx *c, int32_t mode, int32_t nsteps, double endtime, 
                        int32_t *step, double *dts, int32_t *taken)
 {
     if (!c || !time || !dtold || !step) return set_error(BX_ERR_ARG, "bx_step: null argument");
-    if (mode != BX_STEP_EXACT && mode != BX_STEP_FUSED) return set_error(BX_ERR_ARG, "bx_step: unknown mode");
+    if (mode != BX_STEP_EXACT && mode != BX_STEP_FUSED && mode != BX_STEP_GRAPH)
+        return set_error(BX_ERR_ARG, "bx_step: unknown mode");
     BX_HIP(hipSetDevice(c->device));
-    if (c->prm.precision != BX_PREC_FP64 && mode != BX_STEP_FUSED)
+    if (c->prm.precision != BX_PREC_FP64 && mode == BX_STEP_EXACT)
         return set_error(BX_ERR_UNSUPPORTED, "bx_step: the mixed form runs in fused mode only");
+    const bool graph = mode == BX_STEP_GRAPH;
+    if (graph) mode = BX_STEP_FUSED;
     int32_t k = 0;
     int rc = BX_OK;
     if (mode == BX_STEP_FUSED && !c->fused_ready && !c->fused_unsupported) {
@@ -738,7 +829,7 @@ extern "C" int bx_step(bx_ctx *c, int32_t mode, int32_t nsteps, double endtime, 
         rc = BX_OK;
     }
     if (mode == BX_STEP_FUSED && c->fused_ready) {
-        rc = fused_loop(c, nsteps, endtime, time, dtold, step, dts, &k);
+        rc = fused_loop(c, nsteps, endtime, time, dtold, step, dts, &k, graph);
     } else {
         if ((rc = ensure_aux(c))) return rc;
         while (k < nsteps && *time < endtime && *step < 1000000) {
@@ -763,6 +854,7 @@ extern "C" int bx_tune(bx_ctx *c, int32_t y_chunk, int32_t z_chunk, int32_t occu
     c->fused_ly = y_chunk;
     c->fused_lz = z_chunk;
     c->fused_occ = occupancy;
+    drop_step_graphs(c);
     return BX_OK;
 }
 

@@ -52,8 +52,9 @@ __device__ __forceinline__ void wave_max_to(double v, unsigned long long *dst)
 // y-term of fluxtmp, E = areaxz*(fy(2)-fy(1)) (flux.f90:189), in the storage
 // precision (FP64; FP32 in the mixed form, where the fluxes are FP32).
 template <int STAGE, typename S>
-__global__ __launch_bounds__(256) void k_ydir(StageArgs a, S *__restrict__ Ey, int ly)
+__global__ __launch_bounds__(256) void k_ydir(StageArgs a0, S *__restrict__ Ey, int ly)
 {
+    const StageArgs a = with_dt(a0);
     const int lane = threadIdx.x & 63;
     const int w = wave_id();
     const int nx = a.nx, ny = a.ny;
@@ -115,8 +116,9 @@ __global__ __launch_bounds__(256) void k_ydir(StageArgs a, S *__restrict__ Ey, i
 // fluxtmp with the y-term from k_ydir in the reference's summation order and
 // applies the stage update and everything pointwise up to the next stage.
 template <int STAGE, typename S, int OCC, bool PFE>
-__global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a, const S *__restrict__ Ey)
+__global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a0, const S *__restrict__ Ey)
 {
+    const StageArgs a = with_dt(a0);
     const int lane = threadIdx.x & 63;
     const int w = wave_id();
     const int nx = a.nx;
@@ -591,6 +593,132 @@ static int fused_step_t(bx_ctx *c, double dt, S *u0, S *us, S *ps, S *un, bool r
     if ((rc = launch_stage<2, S>(c, b, c->prof_on ? c->ev[4] : nullptr))) return rc;
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[5], c->stream));
     if ((rc = ghosts6<S>(c, un, (S *)nullptr))) return rc;
+    return BX_OK;
+}
+
+// ------------------------------------------------------ graph replays
+// main.f90:266-281 for the next step of a replay, on the device, in the
+// operations of step_dt (bx_capi.hip): the Courant rate of the current state
+// gives dt; the rate is then cleared for the corrector kernel to accumulate
+// the next one.
+__global__ void k_step_dt(DevScalars *s, double cfl)
+{
+    const double rate = __longlong_as_double((long long)s->rate_bits[0]);
+    const int step = s->step + 1;
+    double dt = cfl / rate;  // courant.f90:29
+    dt = dt * 1.0;
+    const double dtold = s->dtold;
+    if (step > 1 && dtold > 0.0) {
+        const double cap = dtold * 1.1;
+        dt = cap < dt ? cap : dt;  // std::min(dt, cap)
+    }
+    s->dtold = dt;
+    if (dt + s->time > s->endtime) dt = s->endtime - s->time;
+    dt = dt * 0.5;
+    s->dt = dt * 2.0;
+    s->step = step;
+    s->rate_bits[0] = 0ull;
+}
+
+// main.f90:544-545 and the step's entry in the chunk's dt log
+__global__ void k_step_end(DevScalars *s, double *log)
+{
+    const double dt = s->dt;
+    s->time = s->time + dt;
+    const int k = s->logk;
+    if (k < kChunkSteps) log[k] = dt;
+    s->logk = k + 1;
+}
+
+// Two fused steps, u0 -> un -> u0, with dt from the device scalars.
+template <typename S>
+static int enqueue_pair(bx_ctx *c, S *u0, S *us, S *ps, S *un)
+{
+    int rc;
+    for (int h = 0; h < 2; h++) {
+        S *src = h == 0 ? u0 : un, *dst = h == 0 ? un : u0;
+        hipLaunchKernelGGL(k_step_dt, dim3(1), dim3(1), 0, c->stream, c->dsc, c->prm.cfl);
+        BX_HIP(hipGetLastError());
+        StageArgs a = base_args(c, 0.0);
+        a.dtp = &c->dsc->dt;
+        a.uin = src;
+        a.uout = us;
+        a.pout = ps;
+        if ((rc = launch_stage<1, S>(c, a, nullptr))) return rc;
+        if ((rc = ghosts6<S>(c, us, ps))) return rc;
+        StageArgs b = base_args(c, 0.0);
+        b.dtp = &c->dsc->dt;
+        b.uin = us;
+        b.pin = ps;
+        b.uold = src;
+        b.uout = dst;
+        b.rate = &c->dsc->rate_bits[0];
+        if ((rc = launch_stage<2, S>(c, b, nullptr))) return rc;
+        if ((rc = ghosts6<S>(c, dst, (S *)nullptr))) return rc;
+        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, c->stream, c->dsc, c->chunk_dts);
+        BX_HIP(hipGetLastError());
+    }
+    return BX_OK;
+}
+
+void drop_step_graphs(bx_ctx *c)
+{
+    for (int i = 0; i < 2; i++) {
+        if (c->sexec[i]) (void)hipGraphExecDestroy(c->sexec[i]);
+        if (c->sgraph[i]) (void)hipGraphDestroy(c->sgraph[i]);
+        c->sexec[i] = nullptr;
+        c->sgraph[i] = nullptr;
+        c->skey[i] = nullptr;
+    }
+}
+
+// The captured pair that starts from the current state buffer (captured on
+// first use; one per buffer parity and precision, two kept).
+static int pair_exec(bx_ctx *c, hipGraphExec_t *out)
+{
+    const bool f32 = c->state_in_32;
+    const void *key = f32 ? (const void *)c->cons32 : (const void *)c->cons;
+    for (int i = 0; i < 2; i++)
+        if (c->sexec[i] && c->skey[i] == key) {
+            *out = c->sexec[i];
+            return BX_OK;
+        }
+    const int slot = c->sexec[0] ? (c->sexec[1] ? 0 : 1) : 0;
+    if (c->sexec[slot]) (void)hipGraphExecDestroy(c->sexec[slot]);
+    if (c->sgraph[slot]) (void)hipGraphDestroy(c->sgraph[slot]);
+    c->sexec[slot] = nullptr;
+    c->sgraph[slot] = nullptr;
+    c->skey[slot] = nullptr;
+    if (!c->chunk_dts) BX_HIP(hipMalloc((void **)&c->chunk_dts, sizeof(double) * kChunkSteps));
+    BX_HIP(hipStreamSynchronize(c->stream));
+    BX_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = f32 ? enqueue_pair<float>(c, c->cons32, c->ustar32, c->pstar32, c->unext32)
+                       : enqueue_pair<double>(c, c->cons, c->ustar, c->pstar, c->unext);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    BX_HIP(e);
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return hip_check(ei, "hipGraphInstantiate");
+    }
+    c->sgraph[slot] = g;
+    c->sexec[slot] = x;
+    c->skey[slot] = key;
+    *out = x;
+    return BX_OK;
+}
+
+int replay_step_pairs(bx_ctx *c, int pairs)
+{
+    hipGraphExec_t x = nullptr;
+    if (int rc = pair_exec(c, &x)) return rc;
+    for (int i = 0; i < pairs; i++) BX_HIP(hipGraphLaunch(x, c->stream));
     return BX_OK;
 }
 

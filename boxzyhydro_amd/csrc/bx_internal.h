@@ -23,6 +23,8 @@ struct DevScalars {
     double endtime;
     int step;                         // main.f90:213
     int active;                       // 1 while the step loop runs (time<endtime)
+    int logk;                         // steps of the current graph chunk (index into its dt log)
+    int pad2;
 };
 
 // flags bits
@@ -85,6 +87,15 @@ struct bx_ctx {
     bool old_stale;    // cons_old not materialised since the last fused step
     int64_t fallbacks;
 
+    // captured pairs of fused steps (BX_STEP_GRAPH), keyed by the state buffer
+    // they start from (the pair returns the state to it)
+    hipGraph_t sgraph[2];
+    hipGraphExec_t sexec[2];
+    const void *skey[2];
+    void *chunk0;          // 5 planes: the state at the start of a replayed chunk (rollback)
+    int64_t graph_steps;   // steps taken from replays (kept, not undone)
+    double *chunk_dts;     // dt of each step of a chunk
+
     bx::DevScalars *dsc;   // device scalars
     bx::DevScalars *hsc;   // pinned host mirror
 
@@ -145,6 +156,11 @@ int launch_calc_gforce(bx_ctx *c);
 int launch_external_grav(bx_ctx *c);
 int launch_state_tab(bx_ctx *c);
 int fused_prepare(bx_ctx *c);
+// BX_STEP_GRAPH: replay `pairs` captured pairs of fused steps from the current
+// state (device dt, flags accumulating); the scalars must be on the device
+int replay_step_pairs(bx_ctx *c, int pairs);
+void drop_step_graphs(bx_ctx *c);
+constexpr int kChunkSteps = 32;  // steps per replayed chunk, at most
 int narrow_cons(bx_ctx *c);
 int widen_cons(bx_ctx *c);
 

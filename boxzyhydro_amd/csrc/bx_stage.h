@@ -44,7 +44,16 @@ struct StageArgs {
     double gam, gm1, seps, srho, vlimit;
     unsigned int *flags;
     unsigned long long *rate;  // stage 2: next step's Courant rate (max)
+    const double *dtp;         // when set, the step's dt is read from here (graph replays), else dt
 };
+
+// The arguments of a launch with dt resolved: a kernel reads it once at entry.
+__device__ __forceinline__ StageArgs with_dt(const StageArgs &a0)
+{
+    StageArgs a = a0;
+    if (a0.dtp) a.dt = *a0.dtp;
+    return a;
+}
 
 // Stage arguments of a context for a step of full dt (bx_fused.hip)
 StageArgs base_args(bx_ctx *c, double dt);

@@ -15,7 +15,7 @@ module boxzy_hip
   implicit none
 
   integer, parameter :: BX_PREC_FP64 = 0, BX_PREC_MIXED = 1
-  integer, parameter :: BX_STEP_EXACT = 0, BX_STEP_FUSED = 1
+  integer, parameter :: BX_STEP_EXACT = 0, BX_STEP_FUSED = 1, BX_STEP_GRAPH = 2
 
   type, bind(C) :: bx_params
      integer(c_int32_t) :: nx, ny, nz

@@ -268,6 +268,12 @@ class HydroGrid:
         self.time = t.value
         return s.value
 
+    def graph_step_count(self) -> int:
+        """Steps taken from graph replays (BX_STEP_GRAPH) so far."""
+        c = ctypes.c_int64()
+        nv.check(nv.lib().bx_graph_step_count(self._h, ctypes.byref(c)), "bx_graph_step_count")
+        return c.value
+
     def fallback_count(self) -> int:
         c = ctypes.c_int64()
         nv.check(nv.lib().bx_fallback_count(self._h, ctypes.byref(c)), "bx_fallback_count")

@@ -23,6 +23,7 @@ BX_PREC_FP64 = 0
 BX_PREC_MIXED = 1
 BX_STEP_EXACT = 0
 BX_STEP_FUSED = 1
+BX_STEP_GRAPH = 2
 BX_GRAV_POTENTIAL = 1
 BX_GRAV_EXTERNAL = 2
 BX_GRAV_COM_SEQUENTIAL = 4
@@ -91,6 +92,7 @@ _SIGS = {
     "bx_step": [_VP, _I32, _I32, ctypes.c_double, _DP, _DP, ctypes.POINTER(_I32), _DP,
                 ctypes.POINTER(_I32)],
     "bx_fallback_count": [_VP, ctypes.POINTER(ctypes.c_int64)],
+    "bx_graph_step_count": [_VP, ctypes.POINTER(ctypes.c_int64)],
     "bx_tune": [_VP, _I32, _I32, _I32],
     "bx_profile": [_VP, _I32, _DP, ctypes.POINTER(ctypes.c_int64)],
     "bx_sync": [_VP],

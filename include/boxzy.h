@@ -94,17 +94,29 @@ int bx_avisc(bx_ctx *ctx, double *qq);            /* avisc.f90:4-26; qq(3,ngrid)
  * mode BX_STEP_EXACT runs the reference call sequence kernel by kernel;
  * BX_STEP_FUSED runs two fused stencil kernels per step and falls back to the
  * exact sequence for any step that meets a low-density cell (rho<1e-4), the
- * speed clamp, or a first step whose ghosts are not yet filled.  Both give
- * results bit-identical to the reference in FP64.
+ * speed clamp, or a first step whose ghosts are not yet filled.
+ * BX_STEP_GRAPH computes what BX_STEP_FUSED computes, bit for bit, with dt
+ * formed on the device: runs of up to 32 steps that cannot reach endtime
+ * (each dt is at most 1.1x the last) are replayed from a captured hipGraph of
+ * two fused steps, with one host synchronisation per run instead of per step.
+ * A run in which a step met an irregular branch is undone and its steps are
+ * taken one at a time as in BX_STEP_FUSED; the first step and the steps near
+ * endtime are always taken one at a time.
+ * All modes give results bit-identical to the reference in FP64.
  * Returns the number of steps taken in *taken. */
 #define BX_STEP_EXACT 0
 #define BX_STEP_FUSED 1
+#define BX_STEP_GRAPH 2
 int bx_step(bx_ctx *ctx, int32_t mode, int32_t nsteps, double endtime, double *time, double *dtold,
             int32_t *step, double *dts, int32_t *taken);
 
 /* Number of steps that bx_step(BX_STEP_FUSED) had to redo with the exact
  * sequence since the context was created. */
 int bx_fallback_count(const bx_ctx *ctx, int64_t *count);
+
+/* Number of steps that bx_step(BX_STEP_GRAPH) took from graph replays (runs
+ * that were undone are not counted) since the context was created. */
+int bx_graph_step_count(const bx_ctx *ctx, int64_t *count);
 
 /* Work split of the fused stage kernels: y rows per wave of the y-direction
  * kernel, z planes per wave of the x/z kernel, and the x/z kernel variant
