@@ -68,28 +68,71 @@ def cpu_baseline(prm, ic, steps: int, threads: int):
                       f"step (oracle/boxzy_oracle.c)"}
 
 
-def timed_run(prm, ic, precision, steps, warmup, tile=None):
-    """(cell-updates/s, ms per step, per-kernel average ms [y1, xz1, y2, xz2], x/z variant, fallbacks)."""
+def timed_run(prm, ic, precision, steps, warmup, tile=None, mode=None):
+    """(cell-updates/s, ms per step, per-kernel average ms [y1, xz1, y2, xz2], launches timed, fallbacks).
+
+    The timed steps run in `mode` (default: graph replays); the per-kernel
+    averages come from `steps` further fused steps timed with HIP events."""
     from boxzyhydro_amd import native as nv
     from boxzyhydro_amd.hydro import HydroGrid
 
+    mode = nv.BX_STEP_GRAPH if mode is None else mode
     g = HydroGrid(prm, ic, device=0, precision=precision)
     if tile:
         g.tune(*tile)
-    g.run(warmup, mode=nv.BX_STEP_FUSED)
+    g.run(warmup, mode=mode, endtime=1e300)
     g.sync()
-    g.profile(1)
     t0 = time.perf_counter()
-    dts = g.run(steps, mode=nv.BX_STEP_FUSED)
+    dts = g.run(steps, mode=mode, endtime=1e300)
     g.sync()
     elapsed = time.perf_counter() - t0
-    ms, nprof = g.profile(0)
     if len(dts) != steps:
         raise RuntimeError(f"took {len(dts)} of {steps} steps")
+    g.profile(1)
+    g.run(steps, mode=nv.BX_STEP_FUSED, endtime=1e300)
+    g.sync()
+    ms, nprof = g.profile(0)
     per = [m / max(nprof, 1) for m in ms]
     fb = g.fallback_count()
     del g
     return prm.interior_cells * steps / elapsed, elapsed * 1e3 / steps, per, nprof, fb
+
+
+def graph_cost(steps_small: int, steps_box: int, n: int):
+    """Wall time per step of the fused step taken one at a time (a host
+    synchronisation per step) and replayed from captured graphs, on the Sod
+    tube (404x8x4 with ghosts, launch-bound) and on the n^3 box."""
+    from boxzyhydro_amd import ics
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+    from boxzyhydro_amd.params import HydroParams, box_params
+
+    sod = HydroParams(nx=404, ny=8, nz=4, dx=0.0025, dy=0.0025, dz=0.0025, gammafix=1.4, cfl=0.5,
+                      small_rho=1e-12, small_eps=1e-40, vlimit=10.0, endtime=1e300,
+                      no_outflow_xl=False, no_outflow_xr=False, no_outflow_yl=False, no_outflow_yr=False,
+                      no_outflow_zl=False, no_outflow_zr=False)
+    out = {}
+    for name, prm, ic, steps in (("sod_404x8x4", sod, ics.sod_ic(sod, axis=0, x0=0.0), steps_small),
+                                 (f"box_{n}^3", box_params(n), None, steps_box)):
+        if ic is None:
+            ic = ics.box_ic(prm, seed=0)
+        row = {}
+        for label, mode in (("one_at_a_time", nv.BX_STEP_FUSED), ("graph", nv.BX_STEP_GRAPH)):
+            g = HydroGrid(prm, ic, device=0)
+            g.run(3, mode=mode)
+            g.sync()
+            t0 = time.perf_counter()
+            d = g.run(steps, mode=mode, record_dt=False)
+            g.sync()
+            el = time.perf_counter() - t0
+            row[label + "_ms_per_step"] = el * 1e3 / steps
+            row[label + "_cell_updates_per_s"] = prm.interior_cells * steps / el
+            if mode == nv.BX_STEP_GRAPH:
+                row["replayed_steps"] = g.graph_step_count()
+            g.close()
+        row["speedup"] = row["one_at_a_time_ms_per_step"] / row["graph_ms_per_step"]
+        out[name] = row
+    return out
 
 
 def mixed_study(prm, ic, steps):
@@ -402,6 +445,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched shock tubes")
     ap.add_argument("--no-boxes", action="store_true", help="skip the batched uneven boxes")
     ap.add_argument("--boxes-only", action="store_true", help="run only the batched uneven boxes")
+    ap.add_argument("--graph-only", action="store_true", help="run only the graph-replay comparison")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay comparison")
     ap.add_argument("--no-boxes-seq", action="store_true",
                     help="batched boxes without the one-grid-at-a-time comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost")
@@ -445,6 +490,9 @@ def main() -> int:
     if args.options_only:
         print(json.dumps({"options": options_cost(args.size, 3)}))
         return 0
+    if args.graph_only:
+        print(json.dumps({"graph": graph_cost(600, 50, args.size)}))
+        return 0
     if args.boxes_only:
         print(json.dumps({"boxes": boxes_cost(512, 3, not args.no_boxes_seq)}))
         return 0
@@ -484,7 +532,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"3-D box {args.size}^3 interior FP64, outflow, gamma=5/3, CFL 0.5, "
                                f"seeded smoothed Gaussian rho/p, rms Mach 0.5; full main-loop step",
-                   "grid": [prm.nx, prm.ny, prm.nz], "mode": args.mode, "fallback_steps": fb},
+                   "grid": [prm.nx, prm.ny, prm.nz], "mode": "fused, replayed from hipGraphs (BX_STEP_GRAPH)",
+                   "fallback_steps": fb},
         "step_hbm_roofline_frac": value * B_STEP_FP64 / HBM_PEAK,
     }
     xz2 = per[3] * 1e-3
@@ -510,6 +559,9 @@ def main() -> int:
                         "k_xz2_hbm_frac": B_XZ2_MIXED * cells / (perm[3] * 1e-3) / HBM_PEAK,
                         "kernel_avg_ms": {"k_ydir<1>": perm[0], "k_xz<1>": perm[1], "k_ydir<2>": perm[2],
                                           "k_xz<2>": perm[3]}, "fallback_steps": fbm}
+    if not args.no_graph:
+        log("graph replays ...")
+        out["graph"] = graph_cost(600, args.steps, args.size)
     if not args.no_batch:
         log("batched shock tubes ...")
         out["batched"] = batched(512, 3)
