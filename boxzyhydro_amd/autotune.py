@@ -56,8 +56,9 @@ def tune_batch(batch, candidates: Iterable[Tuple[int, int]] = BATCH_CANDIDATES, 
     return best, results
 
 
-# the batched-grid kernel holds about 210 registers per lane: at most 8 waves per CU
-BOX_CANDIDATES = [(8, 1), (4, 1), (4, 2)]
+# the 4- and 8-wave builds of the batched-grid kernel hold about 210 registers
+# per lane (8 waves per CU); the 16-wave build is held to 128 and spills
+BOX_CANDIDATES = [(16, 1), (8, 1), (4, 1), (4, 2)]
 
 
 def tune_boxes(batch, candidates: Iterable[Tuple[int, int]] = BOX_CANDIDATES, reps: int = 2,

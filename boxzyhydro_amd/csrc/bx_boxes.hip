@@ -337,6 +337,7 @@ int enqueue(bx_boxes *b)
     // the queue counter is reset by a kernel (not a memset), so the pair can be captured
     hipLaunchKernelGGL(bx::k_boxes_reset, dim3(1), dim3(1), 0, b->stream, b->queue);
     if (b->waves == 4) hipLaunchKernelGGL(bx::k_boxes<4>, dim3(grid), dim3(256), 0, b->stream, a);
+    else if (b->waves == 16) hipLaunchKernelGGL(bx::k_boxes<16>, dim3(grid), dim3(1024), 0, b->stream, a);
     else hipLaunchKernelGGL(bx::k_boxes<8>, dim3(grid), dim3(512), 0, b->stream, a);
     BX_HIP(hipGetLastError());
     return 0;
@@ -572,11 +573,12 @@ int bx_boxes_order(const bx_boxes *b, int32_t *order)
 int bx_boxes_tune(bx_boxes *b, int32_t waves_per_block, int32_t blocks_per_cu)
 {
     if (!b) return set_error(BX_ERR_ARG, "bx_boxes_tune: null handle");
-    // the kernel holds about 210 registers per lane: two waves per SIMD, eight per CU
-    if (waves_per_block != 4 && waves_per_block != 8)
-        return set_error(BX_ERR_ARG, "bx_boxes_tune: waves per workgroup must be 4 or 8");
-    if (blocks_per_cu < 1 || waves_per_block * blocks_per_cu > 8)
-        return set_error(BX_ERR_ARG, "bx_boxes_tune: at most 8 waves per CU (1..2 workgroups of 4, 1 of 8)");
+    // the 4- and 8-wave builds hold about 210 registers per lane (two waves per
+    // SIMD); the 16-wave build is held to 128 and spills to scratch
+    if (waves_per_block != 4 && waves_per_block != 8 && waves_per_block != 16)
+        return set_error(BX_ERR_ARG, "bx_boxes_tune: waves per workgroup must be 4, 8 or 16");
+    if (blocks_per_cu < 1 || waves_per_block * blocks_per_cu > 16)
+        return set_error(BX_ERR_ARG, "bx_boxes_tune: at most 16 waves per CU");
     b->waves = waves_per_block;
     b->blocks_per_cu = blocks_per_cu;
     drop_graph(b);

@@ -219,8 +219,9 @@ int bx_boxes_cells(const bx_boxes *b, int64_t *cells, int64_t *interior_cells);
 /* Queue order (grid indices, largest estimated cost first). */
 int bx_boxes_order(const bx_boxes *b, int32_t *order);
 
-/* Work split: waves per workgroup (4 or 8) and workgroups per compute unit
- * (at most 8 waves per CU).  Used by the autotuner; results do not change. */
+/* Work split: waves per workgroup (4, 8 or 16) and workgroups per compute
+ * unit (at most 16 waves per CU).  Used by the autotuner; results do not
+ * change. */
 int bx_boxes_tune(bx_boxes *b, int32_t waves_per_block, int32_t blocks_per_cu);
 
 /* Keep the first maxlog dt values of every grid (0: off). */

@@ -115,7 +115,7 @@ def test_gpu_uneven_random_grids_match_exact_path():
     ref = _exact_runs(grids, steps)
     b = GridBatch(grids, [t for _, t, _ in ref])
     b.log_dt(int(steps.max()))
-    for split in ((8, 1), (4, 1), (4, 2)):
+    for split in ((16, 1), (8, 1), (4, 1), (4, 2)):
         for graph in (True, False):
             print(f"boxes: split {split} graph {graph}", flush=True)
             b.tune(*split)
