@@ -14,8 +14,10 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 
 
-def build_hip(jobs: int = 4) -> str:
-    subprocess.run(["make", "-s", f"-j{jobs}", "-C", os.path.join(HERE, "csrc")], check=True)
+def build_hip(jobs: int = 8) -> str:
+    """Compile every HIP source (always, also over an existing build: this is
+    the does-it-build check) and link the library."""
+    subprocess.run(["make", "-B", "-s", f"-j{jobs}", "-C", os.path.join(HERE, "csrc")], check=True)
     return os.path.join(HERE, "libboxzy_hip.so")
 
 
