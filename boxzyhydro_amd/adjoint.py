@@ -106,6 +106,15 @@ class AdjointStepper:
     def tune(self, y_chunk: int, z_chunk: int) -> None:
         nv.check(nv.lib().bx_adjoint_tune(self.grid._h, y_chunk, z_chunk), "bx_adjoint_tune")
 
+    def store(self, nbytes: int = -1) -> int:
+        """Device memory the next checkpointed forward may spend on keeping
+        whole segments (-1: half the free memory; 0: one segment, the others
+        recomputed by the backward).  Returns the number of segments backward
+        sweeps have recomputed so far."""
+        r = ctypes.c_int64()
+        nv.check(nv.lib().bx_adjoint_store(self.grid._h, nbytes, ctypes.byref(r)), "bx_adjoint_store")
+        return r.value
+
 
 def hydro_function():
     """The ``torch.autograd.Function`` (built lazily so that importing this

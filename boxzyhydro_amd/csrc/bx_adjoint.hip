@@ -617,7 +617,13 @@ struct bx_adj {
     int nsteps = 0, every = 0;
     std::vector<double> dts;
     double *ckpt = nullptr;  // checkpoints: U at steps 0, every, 2 every, ... (5 planes each)
-    double *seg = nullptr;   // segment store: U_0..U_every (5 planes), U*_s (5), p*_s (1)
+    // segment stores, each U_0..U_every (5 planes), U*_s (5), p*_s (1) of one
+    // segment, and the segment each holds (-1: none)
+    std::vector<double *> stores;
+    std::vector<int> store_seg;
+    size_t store_bytes = 0;     // bytes of each store
+    int ckpt_count = 0;         // checkpoints the checkpoint buffer holds
+    int64_t store_budget = -1;  // bytes for segment stores; -1: half the free device memory
     double *lam[2] = {nullptr, nullptr};
     double *lus = nullptr;  // 5 planes: adjoint of U* (ghost entries zero)
     double *lup = nullptr;  // 5 planes: adjoint of U' (ghost entries zero)
@@ -626,7 +632,7 @@ struct bx_adj {
     int32_t *slot = nullptr;  // per cell: index of its donor entry, or -1
     int ndonor = 0, nfold = 0;
     int chunk_y = 64, chunk_z = 32;
-    bool last_kept = false;  // the segment store holds the last segment's states from the forward
+    int64_t recomputed = 0;  // segments recomputed by backward sweeps
 };
 
 namespace bx {
@@ -635,8 +641,10 @@ void adjoint_free(bx_ctx *c)
 {
     if (!c->adj) return;
     bx_adj *j = c->adj;
-    void *p[] = {j->ckpt, j->seg, j->lam[0], j->lam[1], j->lus, j->lup, j->lw, j->fold, j->slot};
+    void *p[] = {j->ckpt, j->lam[0], j->lam[1], j->lus, j->lup, j->lw, j->fold, j->slot};
     for (void *q : p)
+        if (q) (void)hipFree(q);
+    for (double *q : j->stores)
         if (q) (void)hipFree(q);
     delete j;
     c->adj = nullptr;
@@ -833,16 +841,58 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
     c->fused_valid = true;
     c->aux_stale = false;
     const size_t pb = sizeof(double) * (size_t)c->n;
-    if (j->ckpt) (void)hipFree(j->ckpt);
-    if (j->seg) (void)hipFree(j->seg);
-    j->ckpt = j->seg = nullptr;
+    const int64_t n = c->n;
     j->nsteps = 0;
     j->every = every;
-    if (every > 0) {
-        const int nck = (nsteps + every - 1) / every;
-        BX_HIP(hipMalloc((void **)&j->ckpt, 5 * pb * (size_t)std::max(nck, 1)));
-        BX_HIP(hipMalloc((void **)&j->seg, pb * (size_t)(5 * (every + 1) + 6 * every)));
+    // Checkpoints every `every` steps; the last `nkeep` segments also keep
+    // their per-step states (U, U*, p*) from this forward, as many as the
+    // store budget holds (at least one store, which the backward needs to
+    // recompute the others).  Writing them costs no traffic: the steps write
+    // their stage results there instead of into the scratch planes.
+    const int nseg = every > 0 && nsteps > 0 ? (nsteps + every - 1) / every : 0;
+    const size_t seg_bytes = pb * (size_t)(5 * (every + 1) + 6 * every);
+    int nkeep = 0;
+    if (nseg > 0) {
+        // buffers are kept between forwards while their sizes still fit
+        if (j->ckpt && j->ckpt_count < nseg) {
+            (void)hipFree(j->ckpt);
+            j->ckpt = nullptr;
+        }
+        if (!j->ckpt) {
+            BX_HIP(hipMalloc((void **)&j->ckpt, 5 * pb * (size_t)nseg));
+            j->ckpt_count = nseg;
+        }
+        if (j->store_bytes != seg_bytes) {
+            for (double *q : j->stores)
+                if (q) (void)hipFree(q);
+            j->stores.clear();
+            j->store_bytes = seg_bytes;
+        }
+        int64_t budget = j->store_budget;
+        if (budget < 0) {
+            size_t fr = 0, tot = 0;
+            BX_HIP(hipMemGetInfo(&fr, &tot));
+            budget = (int64_t)((fr + j->stores.size() * seg_bytes) / 2);
+        }
+        nkeep = (int)std::max<int64_t>(1, std::min<int64_t>(nseg, budget / (int64_t)seg_bytes));
+        while ((int)j->stores.size() > nkeep) {
+            (void)hipFree(j->stores.back());
+            j->stores.pop_back();
+        }
+        while ((int)j->stores.size() < nkeep) {
+            double *q = nullptr;
+            const hipError_t e = hipMalloc((void **)&q, seg_bytes);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                if (j->stores.empty()) return hip_check(e, "hipMalloc adjoint segment store");
+                nkeep = (int)j->stores.size();  // fewer stores than the budget allows: recompute the rest
+                break;
+            }
+            j->stores.push_back(q);
+        }
     }
+    j->store_seg.assign(j->stores.size(), -1);
+    const int first_kept = nseg - nkeep;  // segments [first_kept, nseg) are kept
     j->dts.assign(nsteps, 0.0);
     double dtold = 0.0, time = 0.0;
     // with a given dt sequence nothing is read back per step: the flags
@@ -854,18 +904,18 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
         if (f & FLAG_IRREGULAR) return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: " + at + " met rho<1e-4 or |u|>vlimit");
         return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: " + at + " met a floor of the update (not differentiated)");
     };
-    // the last segment's states are written straight into the segment store,
-    // so the first backward needs no recompute for it
-    const int64_t n = c->n;
-    const int s_last = every > 0 && nsteps > 0 ? ((nsteps - 1) / every) * every : nsteps;
-    double *Useg = j->seg, *Sseg = every > 0 ? j->seg + (size_t)5 * n * (every + 1) : nullptr;
-    double *Pseg = every > 0 ? Sseg + (size_t)5 * n * every : nullptr;
-    j->last_kept = false;
+    auto U = [&](int i, int t) { return j->stores[i] + (size_t)5 * n * t; };
+    auto Sx = [&](int i, int t) { return j->stores[i] + (size_t)5 * n * (every + 1) + (size_t)5 * n * t; };
+    auto Px = [&](int i, int t) { return j->stores[i] + (size_t)5 * n * (every + 1) + (size_t)5 * n * every + (size_t)n * t; };
     for (int k = 0; k < nsteps; k++) {
-        if (every > 0 && k % every == 0)
-            BX_HIP(hipMemcpyAsync(j->ckpt + (size_t)(k / every) * 5 * c->n, c->cons, 5 * pb, hipMemcpyDeviceToDevice,
-                                  c->stream));
-        if (k == s_last) BX_HIP(hipMemcpyAsync(Useg, c->cons, 5 * pb, hipMemcpyDeviceToDevice, c->stream));
+        const int sg = every > 0 ? k / every : 0, t = every > 0 ? k % every : 0;
+        if (every > 0 && t == 0)
+            BX_HIP(hipMemcpyAsync(j->ckpt + (size_t)sg * 5 * n, sg > first_kept ? U(sg - 1 - first_kept, every) : c->cons,
+                                  5 * pb, hipMemcpyDeviceToDevice, c->stream));
+        const bool kept = nseg > 0 && sg >= first_kept;
+        if (kept && t == 0)  // the segment's first state: the scratch state, or the previous store's last
+            BX_HIP(hipMemcpyAsync(U(sg - first_kept, 0), sg > first_kept ? U(sg - 1 - first_kept, every) : c->cons,
+                                  5 * pb, hipMemcpyDeviceToDevice, c->stream));
         double dt;
         if (dts_in) dt = dts_in[k];
         else {
@@ -875,11 +925,9 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
             dtold = dt;
             dt = (dt * 0.5) * 2.0;
         }
-        if (k >= s_last) {
-            const size_t t = (size_t)(k - s_last);
-            if ((rc = fused_step_into(c, dt, Useg + 5 * n * t, Sseg + 5 * n * t, Pseg + n * t, Useg + 5 * n * (t + 1),
-                                      !frozen)))
-                return rc;
+        if (kept) {
+            const int i = sg - first_kept;
+            if ((rc = fused_step_into(c, dt, U(i, t), Sx(i, t), Px(i, t), U(i, t + 1), !frozen))) return rc;
         } else if ((rc = fused_step_into(c, dt, c->cons, c->ustar, c->pstar, c->unext, !frozen))) {
             return rc;
         }
@@ -887,19 +935,19 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
             if ((rc = read_flags_rate(c, &flags, &rate))) return rc;
             if (flags & (FLAG_IRREGULAR | FLAG_FLOOR)) return refuse(flags, k);
         }
-        if (k < s_last) std::swap(c->cons, c->unext);
+        if (!kept) std::swap(c->cons, c->unext);
         time = time + dt;
         j->dts[k] = dt;
         if (dts_out) dts_out[k] = dt;
     }
-    if (nsteps > s_last)  // the final state, from the end of the kept segment
-        BX_HIP(hipMemcpyAsync(c->cons, Useg + (size_t)5 * n * (nsteps - s_last), 5 * pb, hipMemcpyDeviceToDevice,
+    if (nkeep > 0)  // the final state, from the end of the last kept segment
+        BX_HIP(hipMemcpyAsync(c->cons, U(nkeep - 1, nsteps - (nseg - 1) * every), 5 * pb, hipMemcpyDeviceToDevice,
                               c->stream));
     if (frozen) {
         if ((rc = read_flags_rate(c, &flags, &rate))) return rc;
         if (flags & (FLAG_IRREGULAR | FLAG_FLOOR)) return refuse(flags, nsteps - 1);
     }
-    j->last_kept = every > 0 && nsteps > s_last;
+    for (int i = 0; i < nkeep; i++) j->store_seg[i] = first_kept + i;
     j->nsteps = nsteps;
     c->aux_stale = true;
     c->old_stale = true;
@@ -910,7 +958,7 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
 extern "C" int bx_adjoint_backward(bx_ctx *c, const double *lam_T, double *lam_0)
 {
     if (!c || !lam_T || !lam_0) return set_error(BX_ERR_ARG, "bx_adjoint_backward: null argument");
-    if (!c->adj || c->adj->every <= 0 || !c->adj->ckpt)
+    if (!c->adj || c->adj->every <= 0 || !c->adj->ckpt || c->adj->stores.empty())
         return set_error(BX_ERR_STATE, "bx_adjoint_backward: no checkpointed forward (bx_adjoint_forward with every > 0)");
     BX_HIP(hipSetDevice(c->device));
     bx_adj *j = c->adj;
@@ -921,19 +969,32 @@ extern "C" int bx_adjoint_backward(bx_ctx *c, const double *lam_T, double *lam_0
     if ((rc = fold(c, j->lam[0], nullptr))) return rc;
     int cur = 0;
     const int E = j->every;
-    double *Useg = j->seg, *Sseg = j->seg + (size_t)5 * n * (E + 1), *Pseg = Sseg + (size_t)5 * n * E;
+    const int ns = (int)j->stores.size();
     for (int s0 = ((j->nsteps - 1) / E) * E; s0 >= 0; s0 -= E) {
-        const int len = std::min(E, j->nsteps - s0);
-        // recompute the segment from its checkpoint with the frozen dts (the
-        // last one is still in the store after the forward)
-        const bool kept = j->last_kept && s0 + len == j->nsteps;
-        j->last_kept = false;
-        if (!kept)
-            BX_HIP(hipMemcpyAsync(Useg, j->ckpt + (size_t)(s0 / E) * 5 * n, 5 * pb, hipMemcpyDeviceToDevice, c->stream));
-        for (int t = 0; t < len && !kept; t++)
-            if ((rc = fused_step_into(c, j->dts[s0 + t], Useg + (size_t)5 * n * t, Sseg + (size_t)5 * n * t,
-                                      Pseg + (size_t)n * t, Useg + (size_t)5 * n * (t + 1), true)))
-                return rc;
+        const int sg = s0 / E, len = std::min(E, j->nsteps - s0);
+        int i = -1;
+        for (int q = 0; q < ns; q++)
+            if (j->store_seg[q] == sg) i = q;
+        if (i < 0) {
+            // recompute the segment from its checkpoint with the frozen dts, into a
+            // store whose segment this sweep has passed, else the one needed last
+            for (int q = 0; q < ns && i < 0; q++)
+                if (j->store_seg[q] < 0 || j->store_seg[q] > sg) i = q;
+            if (i < 0) {
+                i = 0;
+                for (int q = 1; q < ns; q++)
+                    if (j->store_seg[q] < j->store_seg[i]) i = q;
+            }
+            double *Useg = j->stores[i], *Sseg = Useg + (size_t)5 * n * (E + 1), *Pseg = Sseg + (size_t)5 * n * E;
+            BX_HIP(hipMemcpyAsync(Useg, j->ckpt + (size_t)sg * 5 * n, 5 * pb, hipMemcpyDeviceToDevice, c->stream));
+            for (int t = 0; t < len; t++)
+                if ((rc = fused_step_into(c, j->dts[s0 + t], Useg + (size_t)5 * n * t, Sseg + (size_t)5 * n * t,
+                                          Pseg + (size_t)n * t, Useg + (size_t)5 * n * (t + 1), true)))
+                    return rc;
+            j->store_seg[i] = sg;
+            j->recomputed++;
+        }
+        double *Useg = j->stores[i], *Sseg = Useg + (size_t)5 * n * (E + 1), *Pseg = Sseg + (size_t)5 * n * E;
         for (int t = len - 1; t >= 0; t--) {
             if ((rc = adj_step(c, j->dts[s0 + t], Useg + (size_t)5 * n * t, Sseg + (size_t)5 * n * t,
                                Pseg + (size_t)n * t, j->lam[cur], j->lam[1 - cur])))
@@ -954,6 +1015,17 @@ extern "C" int bx_adjoint_tune(bx_ctx *c, int32_t y_chunk, int32_t z_chunk)
     if ((rc = adj_init(c))) return rc;
     c->adj->chunk_y = y_chunk;
     c->adj->chunk_z = z_chunk;
+    return BX_OK;
+}
+
+extern "C" int bx_adjoint_store(bx_ctx *c, int64_t bytes, int64_t *recomputed)
+{
+    if (!c || bytes < -1) return set_error(BX_ERR_ARG, "bx_adjoint_store: bad arguments");
+    BX_HIP(hipSetDevice(c->device));
+    int rc;
+    if ((rc = adj_init(c))) return rc;
+    c->adj->store_budget = bytes;
+    if (recomputed) *recomputed = c->adj->recomputed;
     return BX_OK;
 }
 

@@ -120,6 +120,7 @@ _SIGS = {
     "bx_adjoint_forward": [_VP, _I32, _I32, _DP, _DP],
     "bx_adjoint_backward": [_VP, _VP, _VP],
     "bx_adjoint_tune": [_VP, _I32, _I32],
+    "bx_adjoint_store": [_VP, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)],
     "bx_cons_device": [_VP, _VP, _I32],
     "bx_write_celldump": [_VP, ctypes.c_char_p, ctypes.c_double, _I32, _I32],
     "bx_read_celldump": [_VP, ctypes.c_char_p, _DP, ctypes.POINTER(_I32)],

@@ -262,14 +262,23 @@ int bx_boxes_dts(bx_boxes *b, double *dts);
  * bx_adjoint_backward: lam_T and lam_0 are DEVICE pointers to 5 SoA planes of
  * ngrid doubles (rho, mx, my, mz, E): the adjoint of the final state of the
  * last checkpointed forward, and the resulting adjoint of its initial state
- * (ghost-cell adjoints folded into their donors).  Each segment is recomputed
- * from its checkpoint before its reverse sweep.
+ * (ghost-cell adjoints folded into their donors).  The forward keeps the
+ * per-step states of its last segments in HBM, as many as the store budget
+ * holds (bx_adjoint_store; at least one); every other segment is recomputed
+ * from its checkpoint before its reverse sweep.  Both give the same bits.
  */
 int bx_adjoint_forward(bx_ctx *ctx, int32_t nsteps, int32_t every, const double *dts_in, double *dts_out);
 int bx_adjoint_backward(bx_ctx *ctx, const double *lam_T, double *lam_0);
 
 /* Chunk lengths (cells) of the y- and z-direction reverse kernels; results do not change. */
 int bx_adjoint_tune(bx_ctx *ctx, int32_t y_chunk, int32_t z_chunk);
+
+/* Device memory (bytes) the next bx_adjoint_forward may spend on segment
+ * stores, (5(every+1) + 6 every) planes of ngrid doubles each: -1 (default)
+ * is half the free device memory, 0 keeps one store (every segment but the
+ * last is recomputed).  *recomputed (may be NULL) receives the number of
+ * segments backward sweeps have recomputed so far.  Results do not change. */
+int bx_adjoint_store(bx_ctx *ctx, int64_t bytes, int64_t *recomputed);
 
 /* Copy the FP64 conserved state between the context and a DEVICE buffer of
  * 5 SoA planes of ngrid doubles: to_ctx=1 sets the state (u and p are

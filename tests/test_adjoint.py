@@ -147,6 +147,35 @@ def test_adjoint_deterministic_and_checkpoint_independent():
 
 
 @pytest.mark.gpu
+def test_adjoint_kept_and_recomputed_segments_agree():
+    """The gradient is the same bits whether the forward keeps every segment's
+    states (no recompute), keeps two, or keeps one (each other segment
+    recomputed from its checkpoint, the store reused), and over two backward
+    sweeps of one forward."""
+    import torch
+
+    prm, st, rho0, p0, tgt, interior = _setup(nsteps=11, every=3)  # 4 segments, a partial last one
+    seg_bytes = prm.ngrid * 8 * (5 * (3 + 1) + 6 * 3)
+    st.store(-1)
+    g_all = _grad(st, rho0, p0, tgt, interior)
+    r0 = st.store(-1)
+    assert r0 == 0  # every segment kept: nothing recomputed
+    for nbytes, want in ((2 * seg_bytes, 2), (0, 3)):
+        st.store(nbytes)
+        g = _grad(st, rho0, p0, tgt, interior)
+        r1 = st.store(nbytes)
+        assert r1 - r0 == want, (nbytes, r1 - r0)
+        r0 = r1
+        assert torch.equal(g[0], g_all[0]) and torch.equal(g[1], g_all[1]), nbytes
+    # a second sweep of the same forward (one store: every segment recomputed)
+    lam = (st.forward(rho0, p0, dts=st.dts) - tgt) * interior
+    a = st.backward(lam)
+    b = st.backward(lam)
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+    assert st.store(0) - r0 == 3 + 4
+
+
+@pytest.mark.gpu
 def test_adjoint_full_size_matches_central_differences():
     """The adjoint workload at its full size: 128^3 box, 200 steps, checkpoints
     every 20, loss 0.5|rho_T - rho_target|^2 with the target from a second
