@@ -289,9 +289,11 @@ def adjoint_cost(n: int, nsteps: int, every: int):
                       torch.as_tensor(tic["p"].copy(), device=dev), checkpoint=False)
     del tst
     interior = torch.as_tensor(ics.boundary_flags(prm).reshape(-1) == 0, device=dev)
-    t0 = time.perf_counter()
-    st.forward(rho0, p0)  # Courant dt of every step (read back per step), kept as the frozen sequence
-    t_courant = time.perf_counter() - t0
+    t_courant = float("inf")
+    for _ in range(2):  # Courant dt of every step (formed on the device), kept as the frozen sequence;
+        t0 = time.perf_counter()  # the first call also allocates the checkpoints and segment stores
+        st.forward(rho0, p0)
+        t_courant = min(t_courant, time.perf_counter() - t0)
     dts = st.dts.copy()
     t_fwd = float("inf")
     for _ in range(2):
@@ -327,7 +329,7 @@ def adjoint_cost(n: int, nsteps: int, every: int):
                     "segments as half the free HBM holds (all of them here), so backward = reverse sweep; "
                     "'recompute' keeps one segment and recomputes the others from their checkpoints; "
                     "forward_ms is the same frozen-dt map without checkpoints (no per-step read-back), "
-                    "forward_courant_ms the forward that also forms each step's Courant dt",
+                    "forward_courant_ms the forward that forms each step's Courant dt on the device (with checkpoints)",
             "grad_norm": float(torch.sqrt((g_rho * g_rho).sum() + (g_p * g_p).sum())),
             "finite": bool(np.isfinite(float(g_rho.abs().max())))}
 

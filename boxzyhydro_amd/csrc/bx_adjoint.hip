@@ -633,6 +633,8 @@ struct bx_adj {
     int ndonor = 0, nfold = 0;
     int chunk_y = 64, chunk_z = 32;
     int64_t recomputed = 0;  // segments recomputed by backward sweeps
+    double *dlog = nullptr;  // device log of the Courant dts of a forward
+    int dlog_cap = 0;
 };
 
 namespace bx {
@@ -641,7 +643,7 @@ void adjoint_free(bx_ctx *c)
 {
     if (!c->adj) return;
     bx_adj *j = c->adj;
-    void *p[] = {j->ckpt, j->lam[0], j->lam[1], j->lus, j->lup, j->lw, j->fold, j->slot};
+    void *p[] = {j->ckpt, j->lam[0], j->lam[1], j->lus, j->lup, j->lw, j->fold, j->slot, j->dlog};
     for (void *q : p)
         if (q) (void)hipFree(q);
     for (double *q : j->stores)
@@ -894,15 +896,32 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
     j->store_seg.assign(j->stores.size(), -1);
     const int first_kept = nseg - nkeep;  // segments [first_kept, nseg) are kept
     j->dts.assign(nsteps, 0.0);
-    double dtold = 0.0, time = 0.0;
-    // with a given dt sequence nothing is read back per step: the flags
-    // accumulate over the run and are checked once at its end
+    // Nothing is read back per step: the flags accumulate over the run and are
+    // checked once at its end.  Without a given dt sequence each step's
+    // Courant dt is formed on the device (main.f90:266-281 with no endtime)
+    // and the sequence is read back once.
     const bool frozen = dts_in != nullptr;
-    if (frozen) BX_HIP(hipMemsetAsync(&c->dsc->flags, 0, sizeof(unsigned int), c->stream));
-    auto refuse = [&](unsigned f, int k) -> int {
-        const std::string at = frozen ? std::string("the run") : "step " + std::to_string(k + 1);
-        if (f & FLAG_IRREGULAR) return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: " + at + " met rho<1e-4 or |u|>vlimit");
-        return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: " + at + " met a floor of the update (not differentiated)");
+    if (frozen) {
+        BX_HIP(hipMemsetAsync(&c->dsc->flags, 0, sizeof(unsigned int), c->stream));
+    } else if (nsteps > 0) {
+        if (j->dlog_cap < nsteps) {
+            if (j->dlog) (void)hipFree(j->dlog);
+            j->dlog = nullptr;
+            BX_HIP(hipMalloc((void **)&j->dlog, sizeof(double) * (size_t)nsteps));
+            j->dlog_cap = nsteps;
+        }
+        DevScalars *h = c->hsc;  // rate_bits[0]: the Courant rate of the regular check above
+        h->time = 0.0;
+        h->dtold = 0.0;
+        h->step = 0;
+        h->endtime = std::numeric_limits<double>::infinity();
+        h->flags = 0;
+        h->logk = 0;
+        BX_HIP(hipMemcpyAsync(c->dsc, h, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream));
+    }
+    auto refuse = [&](unsigned f) -> int {
+        if (f & FLAG_IRREGULAR) return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: the run met rho<1e-4 or |u|>vlimit");
+        return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: the run met a floor of the update (not differentiated)");
     };
     auto U = [&](int i, int t) { return j->stores[i] + (size_t)5 * n * t; };
     auto Sx = [&](int i, int t) { return j->stores[i] + (size_t)5 * n * (every + 1) + (size_t)5 * n * t; };
@@ -916,37 +935,27 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
         if (kept && t == 0)  // the segment's first state: the scratch state, or the previous store's last
             BX_HIP(hipMemcpyAsync(U(sg - first_kept, 0), sg > first_kept ? U(sg - 1 - first_kept, every) : c->cons,
                                   5 * pb, hipMemcpyDeviceToDevice, c->stream));
-        double dt;
-        if (dts_in) dt = dts_in[k];
-        else {
-            dt = c->prm.cfl / rate;  // main.f90:266-281 without an endtime
-            dt = dt * 1.0;
-            if (k + 1 > 1 && dtold > 0.0) dt = std::min(dt, dtold * 1.1);
-            dtold = dt;
-            dt = (dt * 0.5) * 2.0;
-        }
-        if (kept) {
-            const int i = sg - first_kept;
-            if ((rc = fused_step_into(c, dt, U(i, t), Sx(i, t), Px(i, t), U(i, t + 1), !frozen))) return rc;
-        } else if ((rc = fused_step_into(c, dt, c->cons, c->ustar, c->pstar, c->unext, !frozen))) {
+        const int i = kept ? sg - first_kept : 0;
+        double *u0 = kept ? U(i, t) : c->cons, *us = kept ? Sx(i, t) : c->ustar;
+        double *ps = kept ? Px(i, t) : c->pstar, *un = kept ? U(i, t + 1) : c->unext;
+        if (frozen) {
+            if ((rc = fused_step_into(c, dts_in[k], u0, us, ps, un, false))) return rc;
+            j->dts[k] = dts_in[k];
+        } else if ((rc = fused_step_device_dt(c, u0, us, ps, un, j->dlog, nsteps))) {
             return rc;
         }
-        if (!frozen) {
-            if ((rc = read_flags_rate(c, &flags, &rate))) return rc;
-            if (flags & (FLAG_IRREGULAR | FLAG_FLOOR)) return refuse(flags, k);
-        }
         if (!kept) std::swap(c->cons, c->unext);
-        time = time + dt;
-        j->dts[k] = dt;
-        if (dts_out) dts_out[k] = dt;
     }
+    if (!frozen && nsteps > 0)
+        BX_HIP(hipMemcpyAsync(j->dts.data(), j->dlog, sizeof(double) * (size_t)nsteps, hipMemcpyDeviceToHost,
+                              c->stream));
     if (nkeep > 0)  // the final state, from the end of the last kept segment
         BX_HIP(hipMemcpyAsync(c->cons, U(nkeep - 1, nsteps - (nseg - 1) * every), 5 * pb, hipMemcpyDeviceToDevice,
                               c->stream));
-    if (frozen) {
-        if ((rc = read_flags_rate(c, &flags, &rate))) return rc;
-        if (flags & (FLAG_IRREGULAR | FLAG_FLOOR)) return refuse(flags, nsteps - 1);
-    }
+    if ((rc = read_flags_rate(c, &flags, &rate))) return rc;  // also completes the dt read-back
+    if (flags & (FLAG_IRREGULAR | FLAG_FLOOR)) return refuse(flags);
+    if (dts_out)
+        for (int k = 0; k < nsteps; k++) dts_out[k] = j->dts[k];
     for (int i = 0; i < nkeep; i++) j->store_seg[i] = first_kept + i;
     j->nsteps = nsteps;
     c->aux_stale = true;

@@ -570,12 +570,14 @@ int widen_cons(bx_ctx *c)
 // accumulates into rate_bits[0] and irregular branches into flags (both
 // zeroed here).
 template <typename S>
-static int fused_step_t(bx_ctx *c, double dt, S *u0, S *us, S *ps, S *un, bool reset_flags = true)
+static int fused_step_t(bx_ctx *c, double dt, S *u0, S *us, S *ps, S *un, bool reset_flags = true,
+                        const double *dtp = nullptr)
 {
     int rc;
     if (reset_flags) BX_HIP(hipMemsetAsync(&c->dsc->flags, 0, sizeof(unsigned int), c->stream));
     BX_HIP(hipMemsetAsync(&c->dsc->rate_bits[0], 0, sizeof(unsigned long long), c->stream));
     StageArgs a = base_args(c, dt);
+    a.dtp = dtp;
     a.uin = u0;
     a.uout = us;
     a.pout = ps;
@@ -584,6 +586,7 @@ static int fused_step_t(bx_ctx *c, double dt, S *u0, S *us, S *ps, S *un, bool r
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[2], c->stream));
     if ((rc = ghosts6<S>(c, us, ps))) return rc;
     StageArgs b = base_args(c, dt);
+    b.dtp = dtp;
     b.uin = us;
     b.pin = ps;
     b.uold = u0;
@@ -620,13 +623,13 @@ __global__ void k_step_dt(DevScalars *s, double cfl)
     s->rate_bits[0] = 0ull;
 }
 
-// main.f90:544-545 and the step's entry in the chunk's dt log
-__global__ void k_step_end(DevScalars *s, double *log)
+// main.f90:544-545 and the step's entry in a dt log of `cap` entries
+__global__ void k_step_end(DevScalars *s, double *log, int cap)
 {
     const double dt = s->dt;
     s->time = s->time + dt;
     const int k = s->logk;
-    if (k < kChunkSteps) log[k] = dt;
+    if (k < cap) log[k] = dt;
     s->logk = k + 1;
 }
 
@@ -655,7 +658,7 @@ static int enqueue_pair(bx_ctx *c, S *u0, S *us, S *ps, S *un)
         b.rate = &c->dsc->rate_bits[0];
         if ((rc = launch_stage<2, S>(c, b, nullptr))) return rc;
         if ((rc = ghosts6<S>(c, dst, (S *)nullptr))) return rc;
-        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, c->stream, c->dsc, c->chunk_dts);
+        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, c->stream, c->dsc, c->chunk_dts, kChunkSteps);
         BX_HIP(hipGetLastError());
     }
     return BX_OK;
@@ -725,6 +728,18 @@ int replay_step_pairs(bx_ctx *c, int pairs)
 int fused_step_into(bx_ctx *c, double dt, double *u0, double *us, double *ps, double *un, bool reset_flags)
 {
     return fused_step_t<double>(c, dt, u0, us, ps, un, reset_flags);
+}
+
+// A step whose dt is formed on the device from the Courant rate of its state
+// (main.f90:266-281, endtime from the device scalars), logged into log[cap].
+int fused_step_device_dt(bx_ctx *c, double *u0, double *us, double *ps, double *un, double *log, int cap)
+{
+    hipLaunchKernelGGL(k_step_dt, dim3(1), dim3(1), 0, c->stream, c->dsc, c->prm.cfl);
+    BX_HIP(hipGetLastError());
+    if (int rc = fused_step_t<double>(c, 0.0, u0, us, ps, un, false, &c->dsc->dt)) return rc;
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, c->stream, c->dsc, log, cap);
+    BX_HIP(hipGetLastError());
+    return BX_OK;
 }
 
 int fused_step(bx_ctx *c, double dt)

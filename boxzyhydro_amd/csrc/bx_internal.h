@@ -149,6 +149,10 @@ int fused_step(bx_ctx *c, double dt);
 // one FP64 fused step U^n=u0 -> U*=us, p*=ps -> U^{n+1}=un (ghosts filled); the rate is zeroed
 // first, the flags too when reset_flags (else they accumulate over steps)
 int fused_step_into(bx_ctx *c, double dt, double *u0, double *us, double *ps, double *un, bool reset_flags);
+// the same with dt formed on the device from the current Courant rate (the
+// device scalars carry time, dtold, step and endtime); flags accumulate and
+// each dt is logged into log[cap]
+int fused_step_device_dt(bx_ctx *c, double *u0, double *us, double *ps, double *un, double *log, int cap);
 void adjoint_free(bx_ctx *c);
 void eos_free(bx_ctx *c);
 void gravity_free(bx_ctx *c);

@@ -147,6 +147,31 @@ def test_adjoint_deterministic_and_checkpoint_independent():
 
 
 @pytest.mark.gpu
+def test_adjoint_courant_forward_matches_step():
+    """The adjoint's forward with its own Courant dts (formed on the device,
+    read back once) is bx_step's fused step from the same regular state: the
+    same dt sequence and final state, bit for bit."""
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+
+    prm, st, rho0, p0, tgt, interior = _setup(nsteps=9, every=4)
+    d_adj = np.asarray(st.dts).copy()
+    final = st.state().cpu().numpy().T.copy()  # (n, 5)
+    # the same regular state in a context of its own: ghosts filled, u = m/rho
+    cons = st.cons_of(rho0, p0).cpu().numpy().T.copy()
+    g = HydroGrid(prm, {"cons": cons, "u": cons[:, 1:4] / cons[:, 0:1], "p": p0.cpu().numpy().copy(),
+                        "adindx": np.full(prm.ngrid, prm.gammafix)})
+    g.set_ghost_cells()
+    c = g.download()["cons"]
+    g.upload({"cons": c, "u": c[:, 1:4] / c[:, 0:1], "p": p0.cpu().numpy().copy(),
+              "adindx": np.full(prm.ngrid, prm.gammafix)})
+    d_step = g.run(9, endtime=1e300, mode=nv.BX_STEP_FUSED)
+    assert np.array_equal(d_adj.view(np.uint64), np.asarray(d_step).view(np.uint64))
+    assert np.array_equal(final.view(np.uint64), g.download()["cons"].view(np.uint64))
+    g.close()
+
+
+@pytest.mark.gpu
 def test_adjoint_kept_and_recomputed_segments_agree():
     """The gradient is the same bits whether the forward keeps every segment's
     states (no recompute), keeps two, or keeps one (each other segment
