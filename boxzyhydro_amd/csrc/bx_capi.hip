@@ -693,7 +693,7 @@ static int graph_chunk(bx_ctx *c, int K, double endtime, double *time, double *d
         h->rate_bits[0] = rate0;
         return BX_OK;
     }
-    *time = h->time;
+    *time = h->time + h->dt;  // the last step's main.f90:545 (the device advanced time for the others)
     *dtold = h->dtold;
     *step = h->step;
     if (dts)

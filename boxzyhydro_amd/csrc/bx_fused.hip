@@ -600,35 +600,36 @@ static int fused_step_t(bx_ctx *c, double dt, S *u0, S *us, S *ps, S *un, bool r
 }
 
 // ------------------------------------------------------ graph replays
-// main.f90:266-281 for the next step of a replay, on the device, in the
-// operations of step_dt (bx_capi.hip): the Courant rate of the current state
-// gives dt; the rate is then cleared for the corrector kernel to accumulate
-// the next one.
-__global__ void k_step_dt(DevScalars *s, double cfl)
+// One step's scalars on the device (graph replays and the adjoint's Courant
+// forward), in the operations of the host driver (step_dt in bx_capi.hip):
+// first the previous step of the run advances time (main.f90:544-545), then
+// the Courant rate of the current state gives dt (main.f90:266-281), which is
+// logged; the rate is cleared for the corrector kernel to accumulate the next
+// one.  The host adds the last step's dt to time after the run.
+__global__ void k_step_dt(DevScalars *s, double cfl, double *log, int cap)
 {
+    double time = s->time;
+    const int k = s->logk;
+    if (k > 0) {
+        time = time + s->dt;
+        s->time = time;
+    }
     const double rate = __longlong_as_double((long long)s->rate_bits[0]);
     const int step = s->step + 1;
     double dt = cfl / rate;  // courant.f90:29
     dt = dt * 1.0;
     const double dtold = s->dtold;
     if (step > 1 && dtold > 0.0) {
-        const double cap = dtold * 1.1;
-        dt = cap < dt ? cap : dt;  // std::min(dt, cap)
+        const double cap_dt = dtold * 1.1;
+        dt = cap_dt < dt ? cap_dt : dt;  // std::min(dt, cap_dt)
     }
     s->dtold = dt;
-    if (dt + s->time > s->endtime) dt = s->endtime - s->time;
+    if (dt + time > s->endtime) dt = s->endtime - time;
     dt = dt * 0.5;
-    s->dt = dt * 2.0;
+    dt = dt * 2.0;
+    s->dt = dt;
     s->step = step;
     s->rate_bits[0] = 0ull;
-}
-
-// main.f90:544-545 and the step's entry in a dt log of `cap` entries
-__global__ void k_step_end(DevScalars *s, double *log, int cap)
-{
-    const double dt = s->dt;
-    s->time = s->time + dt;
-    const int k = s->logk;
     if (k < cap) log[k] = dt;
     s->logk = k + 1;
 }
@@ -640,7 +641,8 @@ static int enqueue_pair(bx_ctx *c, S *u0, S *us, S *ps, S *un)
     int rc;
     for (int h = 0; h < 2; h++) {
         S *src = h == 0 ? u0 : un, *dst = h == 0 ? un : u0;
-        hipLaunchKernelGGL(k_step_dt, dim3(1), dim3(1), 0, c->stream, c->dsc, c->prm.cfl);
+        hipLaunchKernelGGL(k_step_dt, dim3(1), dim3(1), 0, c->stream, c->dsc, c->prm.cfl, c->chunk_dts,
+                           kChunkSteps);
         BX_HIP(hipGetLastError());
         StageArgs a = base_args(c, 0.0);
         a.dtp = &c->dsc->dt;
@@ -658,8 +660,6 @@ static int enqueue_pair(bx_ctx *c, S *u0, S *us, S *ps, S *un)
         b.rate = &c->dsc->rate_bits[0];
         if ((rc = launch_stage<2, S>(c, b, nullptr))) return rc;
         if ((rc = ghosts6<S>(c, dst, (S *)nullptr))) return rc;
-        hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, c->stream, c->dsc, c->chunk_dts, kChunkSteps);
-        BX_HIP(hipGetLastError());
     }
     return BX_OK;
 }
@@ -734,12 +734,9 @@ int fused_step_into(bx_ctx *c, double dt, double *u0, double *us, double *ps, do
 // (main.f90:266-281, endtime from the device scalars), logged into log[cap].
 int fused_step_device_dt(bx_ctx *c, double *u0, double *us, double *ps, double *un, double *log, int cap)
 {
-    hipLaunchKernelGGL(k_step_dt, dim3(1), dim3(1), 0, c->stream, c->dsc, c->prm.cfl);
+    hipLaunchKernelGGL(k_step_dt, dim3(1), dim3(1), 0, c->stream, c->dsc, c->prm.cfl, log, cap);
     BX_HIP(hipGetLastError());
-    if (int rc = fused_step_t<double>(c, 0.0, u0, us, ps, un, false, &c->dsc->dt)) return rc;
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, c->stream, c->dsc, log, cap);
-    BX_HIP(hipGetLastError());
-    return BX_OK;
+    return fused_step_t<double>(c, 0.0, u0, us, ps, un, false, &c->dsc->dt);
 }
 
 int fused_step(bx_ctx *c, double dt)

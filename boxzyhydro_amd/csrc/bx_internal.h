@@ -23,7 +23,7 @@ struct DevScalars {
     double endtime;
     int step;                         // main.f90:213
     int active;                       // 1 while the step loop runs (time<endtime)
-    int logk;                         // steps of the current graph chunk (index into its dt log)
+    int logk;                         // steps of the current device-dt run (index into its dt log)
     int pad2;
 };
 
