@@ -12,7 +12,9 @@ import os
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libboxzy_hip.so")
+# BOXZY_LIB selects another build of the same library (e.g. the host-sanitized
+# one of `make -C boxzyhydro_amd/csrc host-asan`); a missing file still raises
+LIB_PATH = os.environ.get("BOXZY_LIB") or os.path.join(HERE, "libboxzy_hip.so")
 
 BX_OK = 0
 BX_ERR_ARG = -1
