@@ -2,7 +2,7 @@
  * boxzy_oracle.c -- CPU restatement of the BOXZYHYDRO hydro step (oracle).
  *
  * TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
- * CPU-baseline leg of bench.py as the checker.  The product path never calls it.
+ * CPU-baseline leg of bench.py, to check against.  The product path never calls it.
  *
  * It restates, in C with OpenMP, the reference's Fortran subroutines with the
  * same arithmetic order, the same Fortran-ordered arrays (cons(5,ngrid) ...),
