@@ -159,6 +159,42 @@ def test_gpu_endtime_clip_and_queue_order():
     b.close()
 
 
+@pytest.mark.gpu
+def test_gpu_many_small_grids_stop_at_maxsteps():
+    """600 tiny grids (more than the persistent launch's workgroups, so the
+    queue hands out work many times over) stopped by maxsteps before their
+    endtime: every grid takes exactly that many steps, and a sample equals the
+    exact path's run of as many steps."""
+    from boxzyhydro_amd import ics
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.boxes import GridBatch
+    from boxzyhydro_amd.hydro import HydroGrid
+    from boxzyhydro_amd.params import HydroParams
+
+    rng = np.random.Generator(np.random.PCG64(21))
+    grids = []
+    for i in range(600):
+        nx, ny = (int(v) for v in rng.integers(5, 9, 2))
+        nz = int(rng.integers(1, 7))
+        prm = HydroParams(nx=nx, ny=ny, nz=nz, dx=0.25, dy=0.25, dz=0.25, gammafix=1.4, cfl=0.5, endtime=1e30)
+        grids.append((prm, ics.box_ic(prm, seed=5000 + i, sigma_cells=1.0)))
+    b = GridBatch(grids)
+    b.run(maxsteps=4)
+    b.sync()
+    res = b.result()
+    assert all(r["steps"] == 4 and r["status"] == 0 for r in res)
+    for i in range(0, 600, 60):
+        prm, ic = grids[i]
+        g = HydroGrid(prm, ic, device=0)
+        g.run(4, mode=nv.BX_STEP_EXACT)
+        out = g.download()
+        assert res[i]["time"] == g.time, i
+        for k in ("cons", "u", "p"):
+            assert np.array_equal(res[i][k].view(np.uint64), out[k].view(np.uint64)), (i, k)
+        g.close()
+    b.close()
+
+
 def test_create_rejects_unsupported_grids():
     """Checked on the host before any device call."""
     from boxzyhydro_amd import native as nv
