@@ -115,6 +115,34 @@ def test_gpu_graph_run_undone_on_low_density():
     assert ng == 18
 
 
+@pytest.mark.gpu
+def test_gpu_graph_after_tune_and_upload():
+    """Captured pairs are dropped by bx_tune (new tile shapes) and recaptured;
+    an upload keeps the buffers, so the pairs stay valid.  The same calls in
+    fused mode give the same bits."""
+    from boxzyhydro_amd import ics
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+    from boxzyhydro_amd.params import box_params
+
+    prm = box_params(40)
+    ic1, ic2 = ics.box_ic(prm, seed=11), ics.box_ic(prm, seed=12)
+    res = {}
+    for mode in (nv.BX_STEP_FUSED, nv.BX_STEP_GRAPH):
+        g = HydroGrid(prm, ic1)
+        d = [g.run(15, endtime=1e30, mode=mode)]
+        g.tune(8, 8, 0)
+        d.append(g.run(15, endtime=1e30, mode=mode))
+        g.upload(ic2)
+        g.time, g.dtold, g.step_count = 0.0, 0.0, 0
+        d.append(g.run(15, endtime=1e30, mode=mode))
+        res[mode] = (np.concatenate(d), g.download()["cons"], g.graph_step_count())
+        g.close()
+    (df, cf, _), (dg, cg, ng) = res[nv.BX_STEP_FUSED], res[nv.BX_STEP_GRAPH]
+    assert _bits_equal(df, dg) and _bits_equal(cf, cg)
+    assert ng >= 36
+
+
 def test_chunk_bound_keeps_steps_below_endtime():
     """The host bound behind the replayed runs (mirrored here): with each dt
     at most 1.1x the last, K steps from (time, dtold) stay below endtime."""
