@@ -80,7 +80,8 @@ def test_gpu_graph_equals_fused_box64(precision):
 
 
 @pytest.mark.gpu
-def test_gpu_graph_run_undone_on_low_density():
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_graph_run_undone_on_low_density(precision):
     """A diverging flow empties the middle of an outflow box: the density
     there first drops below 1e-4 (the low-density branch of state.f90:52-56,
     which the fused kernels leave to the exact sequence) after step 26.  Steps
@@ -100,9 +101,10 @@ def test_gpu_graph_run_undone_on_low_density():
     rho0 = 0.006
     vx = np.broadcast_to(np.sign(x)[None, None, :] * 1.0, shp)
     ic = ics._assemble(prm, np.full(shp, rho0), vx, np.zeros(shp), np.zeros(shp), np.full(shp, 0.01 * rho0))
+    prec = nv.BX_PREC_FP64 if precision == "fp64" else nv.BX_PREC_MIXED
     res = {}
     for mode in (nv.BX_STEP_FUSED, nv.BX_STEP_GRAPH):
-        g = HydroGrid(prm, ic)
+        g = HydroGrid(prm, ic, precision=prec)
         d = np.concatenate([g.run(20, mode=mode), g.run(30, mode=mode)])
         res[mode] = (d, g.download(), g.fallback_count(), g.graph_step_count(), g.time)
         print(f"mode {mode}: fallbacks {res[mode][2]}, replayed {res[mode][3]}", flush=True)
