@@ -203,10 +203,11 @@ def batched(n_tubes: int, reps: int):
             "status_bits": int(np.bitwise_or.reduce([r["status"] for r in out]))}
 
 
-def box_workload(n_boxes: int, seed: int = 0):
+def box_workload(n_boxes: int, seed: int = 0, walls: bool = False):
     """Uneven 3-D boxes: interior cells per axis log-uniform in [8, 32] (each
-    axis drawn separately), walls on a random subset of sides, seeded smoothed
-    rho/p with rms Mach 0.5, gamma 5/3, CFL 0.5, run to t_end = 2."""
+    axis drawn separately), outflow (or, with ``walls``, walls on a random
+    subset of sides), seeded smoothed rho/p with rms Mach 0.5, gamma 5/3,
+    CFL 0.5, run to t_end = 2."""
     import numpy as np
 
     from boxzyhydro_amd import ics
@@ -216,18 +217,19 @@ def box_workload(n_boxes: int, seed: int = 0):
     out = []
     for i in range(n_boxes):
         nx, ny, nz = (int(round(v)) + 4 for v in np.exp(rng.uniform(np.log(8), np.log(32), 3)))
-        walls = [bool(w) for w in rng.integers(0, 2, 6)]
+        wl = [bool(w) and walls for w in rng.integers(0, 2, 6)]
         prm = HydroParams(nx=nx, ny=ny, nz=nz, dx=0.25, dy=0.25, dz=0.25, gammafix=5.0 / 3.0, cfl=0.5,
-                          endtime=2.0, no_outflow_xl=walls[0], no_outflow_xr=walls[1], no_outflow_yl=walls[2],
-                          no_outflow_yr=walls[3], no_outflow_zl=walls[4], no_outflow_zr=walls[5])
+                          endtime=2.0, no_outflow_xl=wl[0], no_outflow_xr=wl[1], no_outflow_yl=wl[2],
+                          no_outflow_yr=wl[3], no_outflow_zl=wl[4], no_outflow_zr=wl[5])
         out.append((prm, ics.box_ic(prm, seed=seed * 100000 + i, sigma_cells=2.0)))
     return out
 
 
-def boxes_cost(n_boxes: int, reps: int, one_at_a_time: bool = True):
+def boxes_cost(n_boxes: int, reps: int, one_at_a_time: bool = True, walls: bool = False):
     """The uneven-box workload in one persistent launch replayed from a
     captured hipGraph (work split picked by the autotuner), against the same
-    boxes run one after another through the one-kernel-per-subroutine path."""
+    boxes run one after another with bx_step's fused mode (the same bits; the
+    exact sequence where a grid needs it)."""
     import numpy as np
 
     from boxzyhydro_amd import native as nv
@@ -235,7 +237,7 @@ def boxes_cost(n_boxes: int, reps: int, one_at_a_time: bool = True):
     from boxzyhydro_amd.boxes import GridBatch
     from boxzyhydro_amd.hydro import HydroGrid
 
-    grids = box_workload(n_boxes)
+    grids = box_workload(n_boxes, walls=walls)
     b = GridBatch(grids)
     best, res = tune_boxes(b, reps=1)
     b.run(maxsteps=100000, graph=True)
@@ -251,12 +253,13 @@ def boxes_cost(n_boxes: int, reps: int, one_at_a_time: bool = True):
         g = HydroGrid(prm, ic, device=0)
         g.sync()
         t0 = time.perf_counter()
-        g.run(100000, mode=nv.BX_STEP_EXACT, record_dt=False)
+        g.run(100000, mode=nv.BX_STEP_FUSED, record_dt=False)
         g.sync()
         seq_s += time.perf_counter() - t0
         g.close()
-    return {"workload": f"{n_boxes} uneven 3-D boxes, interior per axis log-uniform in [8,32], random walls, "
-                        f"gamma 5/3, CFL 0.5, t_end 2, FP64, one persistent launch replayed from a hipGraph",
+    return {"workload": f"{n_boxes} uneven 3-D boxes, interior per axis log-uniform in [8,32], "
+                        f"{'random walls' if walls else 'outflow'}, gamma 5/3, CFL 0.5, t_end 2, FP64, "
+                        f"one persistent launch replayed from a hipGraph",
             "ms": ms, "cells_interior": int(interior.sum()), "steps_min": int(steps.min()),
             "steps_max": int(steps.max()), "cell_updates_per_s": updates / (ms * 1e-3),
             "one_grid_at_a_time_s": seq_s if one_at_a_time else None,
@@ -462,6 +465,8 @@ def main() -> int:
     ap.add_argument("--graph-only", action="store_true", help="run only the graph-replay comparison")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint/forward cost")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay comparison")
+    ap.add_argument("--boxes-walls", action="store_true",
+                    help="with --boxes-only, also the same boxes with random walls (exact sequence only)")
     ap.add_argument("--no-boxes-seq", action="store_true",
                     help="batched boxes without the one-grid-at-a-time comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost")
@@ -512,7 +517,10 @@ def main() -> int:
         print(json.dumps({"graph": graph_cost(600, 50, args.size)}))
         return 0
     if args.boxes_only:
-        print(json.dumps({"boxes": boxes_cost(512, 3, not args.no_boxes_seq)}))
+        out = {"boxes": boxes_cost(512, 3, not args.no_boxes_seq)}
+        if args.boxes_walls:
+            out["boxes_walls"] = boxes_cost(512, 3, not args.no_boxes_seq, walls=True)
+        print(json.dumps(out))
         return 0
     prm = box_params(args.size)
     log(f"generating box {args.size}^3 ...")

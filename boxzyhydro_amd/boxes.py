@@ -6,7 +6,9 @@ nz < 6 -- run in one persistent launch, optionally replayed from a captured
 hipGraph.  Each grid has its own parameters and endtime and returns what the
 reference's main loop (main.f90:211-555) returns for it in a process of its
 own, bit for bit: the fixed-gamma EOS (H2STAT=-1), the plain flux
-(fluxangmom=0) and the force of a zero potential, in FP64.
+(fluxangmom=0) and the force of a zero potential, in FP64.  A grid steps as
+``bx_step(BX_STEP_FUSED)`` steps it: fused steps on a regular state, the
+exact call sequence where required.
 """
 from __future__ import annotations
 

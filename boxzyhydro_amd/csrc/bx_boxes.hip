@@ -4,21 +4,27 @@
 // Each grid is what the reference runs in one process: main.f90:211-555 on
 // an nx x ny x nz grid with its own walls, spacings, gamma, CFL number and
 // endtime, with the fixed-gamma EOS (H2STAT=-1), the plain flux (no
-// flux_ang) and the uniform force of a zero potential.  One workgroup
-// advances one grid through the reference's call sequence, subroutine by
-// subroutine, with the per-cell bodies of the one-kernel-per-subroutine path
-// (bx_cells.h): the results are bit-identical to that path, and so to the
-// reference.  A subroutine that reads neighbours (the ghost fill, the face
-// fluxes, the stage update) is separated from its producer by a workgroup
-// barrier; the pointwise ones (state, velocity, courant, source, the
-// cons_old copy) run back to back on the same cell-to-lane mapping.  The
-// Courant maximum is a workgroup reduction and the dt logic of
-// main.f90:266-281 runs in every lane on that uniform value.
+// flux_ang) and the uniform force of a zero potential.  One workgroup steps
+// one grid as the single-grid step (bx_step, fused mode) steps it:
+//   - the exact call sequence, subroutine by subroutine, with the per-cell
+//     bodies of the one-kernel-per-subroutine path (bx_cells.h), for the first
+//     step and for any step from a state that is not regular;
+//   - on a regular state whose ghost donors are all interior cells, fused
+//     steps: the wave bodies of the fused stage kernels (bx_waves.h) over the
+//     grid, the workgroup's waves taking the virtual waves of those kernels'
+//     grids; a step that meets an irregular branch is redone with the exact
+//     sequence from U^n.
+// Both give the single-grid step's bits, and so the reference's.  A pass
+// that reads neighbours (ghost fills, faces, stage updates) is separated from
+// its producer by a workgroup barrier; pointwise passes run back to back on
+// the same cell-to-lane mapping.  The Courant maximum is a workgroup
+// reduction and the dt logic of main.f90:266-281 runs in every lane on that
+// uniform value.
 //
 // Work decomposition: grids are taken from a queue ordered by estimated cost
 // (cells x steps, largest first), so the long runs start first and the short
-// ones fill in behind them.  A grid's planes (about 250 B per cell) are its
-// own contiguous region; a grid of up to ~16k cells stays in its XCD's L2.
+// ones fill in behind them.  A grid's planes (240 B per cell) are its own
+// contiguous region; a grid of up to ~16k cells stays in its XCD's L2.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -28,6 +34,7 @@
 #include <vector>
 
 #include "bx_cells.h"
+#include "bx_waves.h"
 
 namespace bx {
 
@@ -51,6 +58,11 @@ struct Box {
     double dx, dy, dz, gam, cfl, endtime, srho, seps, vlimit, muc, rgas;
     double g0, g1, g2;
     double ayz, axz, axy, vol;
+    // fused steps (every ghost donor an interior cell): the stage constants
+    // as the single-grid step forms them (base_args in bx_fused.hip)
+    int fusable, zlo, zhi, ly, lz, vol_pow2;
+    int ds_pow2[3];
+    double inv_ds[3], inv_vol, gm1;
 };
 
 struct BoxArgs {
@@ -88,6 +100,17 @@ __device__ __forceinline__ int64_t opaque_uniform(int64_t v)
     return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
+// A value every lane of the wave holds (read from LDS after a barrier), marked
+// uniform for the compiler so that the step loop's control flow and the stage
+// arguments stay in scalar registers.
+__device__ __forceinline__ double uniform_double(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // one set_ghost_cells call (utils.f90:305-480) on the grid's cons planes,
 // entered and left with every lane's writes visible to the workgroup
 __device__ __forceinline__ void box_ghosts(const Box &b, double *cons, double *scratch, const int32_t *dst,
@@ -115,8 +138,10 @@ __global__ __launch_bounds__(64 * W) void k_boxes(BoxArgs a)
 {
     __shared__ double red[W];
     __shared__ int cur;
+    __shared__ unsigned int sflags;       // irregular branches met by a fused step or the regularity check
+    __shared__ unsigned long long srate;  // Courant rate (IEEE bits) from the corrector or the check
     const int tid = threadIdx.x, nt = 64 * W;
-    const int lane = tid & 63, wv = tid >> 6;
+    const int lane = tid & 63, wv = wave_id();  // wave-uniform for the compiler
     for (;;) {
         __syncthreads();
         if (tid == 0) cur = atomicAdd(a.queue, 1);
@@ -150,7 +175,124 @@ __global__ __launch_bounds__(64 * W) void k_boxes(BoxArgs a)
         double time = 0.0, dtold = 0.0;
         int step = 0;
         unsigned st = 0;
+        // Fused steps as bx_step(BX_STEP_FUSED) takes them: on a regular state
+        // (after an exact step, ghosts filled, rho >= 1e-4, u = m/rho), two
+        // stages of the fused kernels' wave bodies over this grid; a step that
+        // meets the low-density branch or the clamp is redone with the exact
+        // sequence from U^n.  Buffers of the fused step reuse cons_old and the
+        // face-flux planes: U* = old, p* = F[0], U^{n+1} = F[1..5], y-term = F[6..10].
+        bool regular = false, aux_stale = false;
+        double rate_f = 0.0;
+        double *us = old, *ps = F, *un = F + n, *ey = F + 6 * n;
         while (step < b.maxsteps && time < b.endtime) {  // main.f90:212
+            double dt_step = 0.0;
+            bool fused_done = false;
+            if (b.fusable && regular) {
+                // main.f90:266-281 on the rate of the current state
+                double dt = b.cfl / rate_f;
+                dt = dt * 1.0;
+                if (step + 1 > 1 && dtold > 0.0) {
+                    const double cap = dtold * 1.1;
+                    dt = cap < dt ? cap : dt;
+                }
+                const double dtcap = dt;
+                if (dt + time > b.endtime) dt = b.endtime - time;
+                if (!(dt > 0.0) || !(dt <= 1.7976931348623157e308)) {
+                    st |= BOX_BADDT;
+                    break;
+                }
+                dt = dt * 0.5;
+                dt = dt * 2.0;
+                StageArgs sa = {};
+                sa.n = n;
+                sa.nx = b.nx;
+                sa.ny = b.ny;
+                sa.nz = b.nz;
+                sa.flatz = b.flatz;
+                sa.zlo = b.zlo;
+                sa.zhi = b.zhi;
+                sa.lz = b.lz;
+                sa.dt = dt;
+                sa.ds[0] = b.dx;
+                sa.ds[1] = b.dy;
+                sa.ds[2] = b.dz;
+                for (int d = 0; d < 3; d++) {
+                    sa.inv_ds[d] = b.inv_ds[d];
+                    sa.ds_pow2[d] = b.ds_pow2[d];
+                }
+                sa.ayz = b.ayz;
+                sa.axz = b.axz;
+                sa.axy = b.axy;
+                sa.vol = b.vol;
+                sa.inv_vol = b.inv_vol;
+                sa.vol_pow2 = b.vol_pow2;
+                sa.gam = b.gam;
+                sa.gm1 = b.gm1;
+                sa.seps = b.seps;
+                sa.srho = b.srho;
+                sa.vlimit = b.vlimit;
+                sa.flags = &sflags;
+                if (tid == 0) {
+                    sflags = 0u;
+                    srate = 0ull;
+                }
+                __syncthreads();
+                const int nzi = b.zhi - b.zlo;
+                const unsigned ygx = (unsigned)((b.nx - 4 + 63) / 64), ygy = (unsigned)((nzi + 3) / 4);
+                const unsigned ygz = (unsigned)((b.ny - 4 + b.ly - 1) / b.ly);
+                const unsigned xgx = (unsigned)((((int64_t)(b.ny - 4) * b.nx + kSegOut - 1) / kSegOut + 3) / 4);
+                const unsigned xgz = (unsigned)((nzi + b.lz - 1) / b.lz);
+                for (int stage = 1; stage <= 2; stage++) {
+                    if (stage == 1) {
+                        sa.uin = cons;
+                        sa.uout = us;
+                        sa.pout = ps;
+                    } else {
+                        sa.uin = us;
+                        sa.pin = ps;
+                        sa.uold = cons;
+                        sa.uout = un;
+                        sa.pout = nullptr;
+                        sa.rate = &srate;
+                    }
+                    // the y-direction waves, then the x/z waves (each wave of the
+                    // workgroup takes virtual waves of the stage kernels' grids)
+                    for (unsigned v = (unsigned)wv; v < ygx * ygy * ygz * 4u; v += (unsigned)W) {
+                        const unsigned blk = v >> 2;
+                        if (stage == 1) ydir_wave<1, double>(sa, ey, b.ly, blk % ygx, (blk / ygx) % ygy, blk / (ygx * ygy), (int)(v & 3u));
+                        else ydir_wave<2, double>(sa, ey, b.ly, blk % ygx, (blk / ygx) % ygy, blk / (ygx * ygy), (int)(v & 3u));
+                    }
+                    __syncthreads();
+                    for (unsigned v = (unsigned)wv; v < xgx * xgz * 4u; v += (unsigned)W) {
+                        const unsigned blk = v >> 2;
+                        if (stage == 1) xz_wave<1, double, false>(sa, ey, blk % xgx, blk / xgx, (int)(v & 3u));
+                        else xz_wave<2, double, false>(sa, ey, blk % xgx, blk / xgx, (int)(v & 3u));
+                    }
+                    __syncthreads();
+                    for (int64_t k0 = tid; k0 < b.ng; k0 += nt)
+                        ghost6_one<double>(n, opaque(k0), gd, gs, gg, stage == 1 ? us : un, stage == 1 ? ps : nullptr);
+                    __syncthreads();
+                }
+                if ((__builtin_amdgcn_readfirstlane(sflags) & FLAG_IRREGULAR) == 0u) {
+                    BOX_CELLS(c)
+                    for (int f = 0; f < 5; f++) cons[f * n + c] = un[f * n + c];
+                    __syncthreads();
+                    step += 1;
+                    dtold = dtcap;
+                    rate_f = uniform_double(__longlong_as_double((long long)srate));
+                    aux_stale = true;
+                    fused_done = true;
+                    dt_step = dt;
+                } else {
+                    regular = false;  // redo this step with the exact sequence from U^n
+                }
+            }
+            if (!fused_done) {
+            if (aux_stale) {  // u, p, adindx of the last fused step's state (main.f90:513-515)
+                BOX_CELLS(c) materialize_cell(n, c, cons, u, p, ad, b.gam, b.seps);
+                __syncthreads();
+                aux_stale = false;
+            }
             // state (main.f90:232) and the Courant maximum (main.f90:241)
             double r = 0.0;
             BOX_CELLS(c)
@@ -163,6 +305,7 @@ __global__ __launch_bounds__(64 * W) void k_boxes(BoxArgs a)
             __syncthreads();
             double rate = red[0];
             for (int k = 1; k < W; k++) rate = vmax(rate, red[k]);
+            rate = uniform_double(rate);
             // main.f90:266-281, the operations of the host driver
             double dt = b.cfl / rate;
             dt = dt * 1.0;
@@ -223,8 +366,31 @@ __global__ __launch_bounds__(64 * W) void k_boxes(BoxArgs a)
                 velocity_cell(n, c, cons, u, b.vlimit, b.seps);
             }
             dt = dt * 2.0;
-            time = time + dt;  // main.f90:545
-            if (tid == 0 && step - 1 < a.maxlog) a.dtlog[(int64_t)bi * a.maxlog + step - 1] = dt;
+            dt_step = dt;
+            regular = false;
+            if (b.fusable) {
+                // regularity and Courant rate of the new state (k_regular of the single-grid step)
+                if (tid == 0) {
+                    sflags = 0u;
+                    srate = 0ull;
+                }
+                __syncthreads();
+                double rr = 0.0;
+                unsigned flg = 0;
+                BOX_CELLS(c) rr = vmax(rr, regular_cell(n, c, cons, u, b.gam, b.seps, b.dx, b.dy, b.dz, flg));
+                if (flg) atomicOr(&sflags, flg);
+                wave_max_to(rr, &srate);
+                __syncthreads();
+                regular = __builtin_amdgcn_readfirstlane(sflags) == 0u;
+                rate_f = uniform_double(__longlong_as_double((long long)srate));
+            }
+            }
+            time = time + dt_step;  // main.f90:545
+            if (tid == 0 && step - 1 < a.maxlog) a.dtlog[(int64_t)bi * a.maxlog + step - 1] = dt_step;
+        }
+        if (aux_stale) {
+            BOX_CELLS(c) materialize_cell(n, c, cons, u, p, ad, b.gam, b.seps);
+            __syncthreads();
         }
         if (tid == 0) {
             a.t_out[bi] = time;
@@ -278,6 +444,13 @@ double rgas_units()
     const double time = (3600.0 * 365.25 * 24.0) / (2.0 * pi);
     const double vel = 1.496e13 / time;
     return 8.254e7 / ((vel * vel) / 1.0);
+}
+
+// d a power of two (then x/d == x*(1/d) exactly)
+bool is_pow2(double d)
+{
+    int e;
+    return d > 0 && std::frexp(d, &e) == 0.5;
 }
 
 bool check_box(const bx_params &p, std::string &why)
@@ -447,6 +620,20 @@ int bx_boxes_create(int32_t nbox, const bx_params *prm, const double *endtime, c
         b.ayz = q.dy * q.dz;
         b.axz = q.dx * q.dz;
         b.vol = q.dx * q.dy * q.dz;
+        // fused steps need every ghost donor to be an interior cell (one fill pass)
+        b.fusable = ns == 0 ? 1 : 0;
+        b.zlo = b.flatz ? 0 : 2;
+        b.zhi = b.flatz ? q.nz : q.nz - 2;
+        b.ly = 16;
+        b.lz = 16;
+        const double ds[3] = {q.dx, q.dy, q.dz};
+        for (int d = 0; d < 3; d++) {
+            b.inv_ds[d] = 1.0 / ds[d];
+            b.ds_pow2[d] = is_pow2(ds[d]) ? 1 : 0;
+        }
+        b.inv_vol = 1.0 / b.vol;
+        b.vol_pow2 = is_pow2(b.vol) ? 1 : 0;
+        b.gm1 = q.gammafix - 1.0;
         const int64_t in_x = q.nx - 4, in_y = q.ny - 4, in_z = q.nz < 6 ? q.nz : q.nz - 4;
         const int64_t off = cells;  // first cell of this grid in the concatenated host arrays
         cells += b.n;

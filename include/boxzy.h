@@ -192,10 +192,10 @@ int bx_batch_dts(bx_batch *b, double *dts);
 
 /* ------------------------------------------------------------------------
  * Batched grids.  Many independent grids of uneven size (3-D boxes, or flat-z
- * slabs with nz < 6) in one persistent launch: one workgroup runs one grid
- * through the call sequence of main.f90:211-555 (the one-kernel-per-
- * subroutine path's per-cell code, subroutine by subroutine), taking grids
- * from a queue ordered by estimated cost, largest first.  No reference
+ * slabs with nz < 6) in one persistent launch: one workgroup steps one grid
+ * as bx_step(BX_STEP_FUSED) does (the exact call sequence of main.f90:211-555
+ * where required, fused steps on a regular state with shallow ghosts),
+ * taking grids from a queue ordered by estimated cost, largest first.  No reference
  * counterpart to cite: the reference runs one grid per process, and each grid
  * here returns what that process returns, bit for bit.  Every grid has its
  * own bx_params (size, spacings, walls, gammafix, cfl, small_rho/eps,
