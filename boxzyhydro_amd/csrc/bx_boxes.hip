@@ -624,8 +624,8 @@ int bx_boxes_create(int32_t nbox, const bx_params *prm, const double *endtime, c
         b.fusable = ns == 0 ? 1 : 0;
         b.zlo = b.flatz ? 0 : 2;
         b.zhi = b.flatz ? q.nz : q.nz - 2;
-        b.ly = 16;
-        b.lz = 16;
+        b.ly = 32;  // rows and planes per virtual wave (32x32 measured fastest of 8..32)
+        b.lz = 32;
         const double ds[3] = {q.dx, q.dy, q.dz};
         for (int d = 0; d < 3; d++) {
             b.inv_ds[d] = 1.0 / ds[d];
