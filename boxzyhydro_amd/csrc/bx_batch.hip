@@ -207,7 +207,7 @@ __global__ __launch_bounds__(64 * W) void k_tubes(BatchArgs a, const Tube *__res
     __shared__ double red[W];
     __shared__ int cur;
     __shared__ unsigned sflag;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = wave_id();  // wave-uniform for the compiler
     const int64_t T = a.total;
     for (;;) {
         __syncthreads();
