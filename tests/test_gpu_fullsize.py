@@ -3,8 +3,8 @@
 * 256^3 box, 100 steps (the 3-D box workload: outflow, gamma=5/3, smoothed
   seeded density/pressure, rms Mach 0.5) against the reference's own OpenMP
   step (tests/golden/box256.json, made by tests/golden/make_box256_golden.py
-  with oracle/_ref): every dt and the final cons/u/p bit-identical, in both
-  step modes.
+  with oracle/_ref): every dt and the final cons/u/p bit-identical, in all
+  three step modes (exact, fused, graph replays).
 * 512^3 box (137 M cells with ghosts, 64-bit plane offsets beyond 2^30
   elements): the fused step equals the exact call sequence bit for bit.
 * 384^3 box, 100 steps of the mixed form against FP64: rho and p L1 <= 1e-6.
@@ -29,7 +29,7 @@ def test_gpu_box256_100_steps_match_reference():
 
     rec = json.load(open(BOX256))["box256_100"]
     prm, ic = case_inputs(rec)
-    for mode in (nv.BX_STEP_FUSED, nv.BX_STEP_EXACT):
+    for mode in (nv.BX_STEP_FUSED, nv.BX_STEP_GRAPH, nv.BX_STEP_EXACT):
         print(f"box 256^3 x {rec['steps']} steps, mode {mode}", flush=True)
         g = HydroGrid(prm, ic, device=0)
         dts = g.run(rec["steps"], mode=mode)
@@ -105,6 +105,15 @@ def test_gpu_mixed_384_100_steps_error_budget():
         g.run(100, mode=nv.BX_STEP_FUSED)
         out[name] = (g.download(), g.time)
         g.close()
+    # the mixed form replayed from graphs: the same bits as taken one step at a time
+    g = HydroGrid(prm, ic, device=0, precision=nv.BX_PREC_MIXED)
+    g.run(100, mode=nv.BX_STEP_GRAPH)
+    assert g.graph_step_count() > 0
+    og = g.download()
+    assert g.time == out["mixed"][1]
+    for k in ("cons", "p"):
+        assert np.array_equal(og[k].view(np.uint64), out["mixed"][0][k].view(np.uint64)), k
+    g.close()
     interior = ics.boundary_flags(prm).reshape(-1) == 0
     a, b = out["fp64"][0], out["mixed"][0]
     for name, x, y in (("rho", a["cons"][:, 0], b["cons"][:, 0]), ("p", a["p"], b["p"])):
