@@ -82,6 +82,8 @@ class GridBatch:
         nv.check(nv.lib().bx_boxes_tune(self._h, waves_per_block, blocks_per_cu), "bx_boxes_tune")
 
     def log_dt(self, maxlog: int) -> None:
+        """Keep the first `maxlog` dt of each grid in the next runs (0: no log)."""
+        self.maxlog = 0  # the library turns the log off before it allocates the new one
         nv.check(nv.lib().bx_boxes_log_dt(self._h, maxlog), "bx_boxes_log_dt")
         self.maxlog = maxlog
 

@@ -777,13 +777,18 @@ int bx_boxes_log_dt(bx_boxes *b, int32_t maxlog)
     if (!b || maxlog < 0) return set_error(BX_ERR_ARG, "bx_boxes_log_dt: bad argument");
     BX_HIP(hipSetDevice(b->device));
     BX_HIP(hipStreamSynchronize(b->stream));
+    // logging off and the captured launch (which holds the old log's address)
+    // dropped first, so a failed allocation below leaves a handle that runs
+    // without a log
+    drop_graph(b);
+    b->maxlog = 0;
     if (b->dtlog) BX_HIP(hipFree(b->dtlog));
     b->dtlog = nullptr;
-    const size_t cnt = (size_t)std::max(1, maxlog) * (size_t)b->nbox;
+    if (maxlog == 0) return BX_OK;
+    const size_t cnt = (size_t)maxlog * (size_t)b->nbox;
     BX_HIP(hipMalloc((void **)&b->dtlog, sizeof(double) * cnt));
     BX_HIP(hipMemset(b->dtlog, 0, sizeof(double) * cnt));
     b->maxlog = maxlog;
-    drop_graph(b);
     return BX_OK;
 }
 

@@ -160,6 +160,39 @@ def test_gpu_endtime_clip_and_queue_order():
 
 
 @pytest.mark.gpu
+def test_gpu_dt_log_on_off_between_graph_runs():
+    """bx_boxes_log_dt between replays of the captured launch: the log is
+    turned off (dts refused) and on again with another length, the launch is
+    recaptured each time, and every run returns the same bits."""
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.boxes import GridBatch
+
+    grids = _random_grids(seed=31, count=3)
+    b = GridBatch(grids)
+    states, logs = [], []
+    for maxlog in (6, 0, 3):
+        b.log_dt(maxlog)
+        b.run(maxsteps=6, graph=True)
+        b.sync()
+        res = b.result()
+        assert all(r["steps"] == 6 for r in res)
+        states.append([r["cons"].copy() for r in res])
+        if maxlog:
+            dts = b.dts()
+            assert [len(d) for d in dts] == [maxlog] * 3
+            logs.append(dts)
+        else:
+            with pytest.raises(nv.BoxzyError):
+                b.dts()
+    for run in states[1:]:
+        for x, y in zip(states[0], run):
+            assert np.array_equal(x.view(np.uint64), y.view(np.uint64))
+    for x, y in zip(logs[0], logs[1]):
+        assert np.array_equal(x[:3].view(np.uint64), y.view(np.uint64))
+    b.close()
+
+
+@pytest.mark.gpu
 def test_gpu_many_small_grids_stop_at_maxsteps():
     """600 tiny grids (more than the persistent launch's workgroups, so the
     queue hands out work many times over) stopped by maxsteps before their
