@@ -831,6 +831,7 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
     }
     if ((rc = adj_init(c))) return rc;
     bx_adj *j = c->adj;
+    j->nsteps = 0;  // set again only when this forward completes: a refused run leaves nothing to sweep
     // a regular state: ghosts filled from their donors, u = m/rho, p = state()
     if ((rc = launch_ghosts(c, c->cons))) return rc;
     if ((rc = launch_materialize(c))) return rc;
@@ -844,7 +845,6 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
     c->aux_stale = false;
     const size_t pb = sizeof(double) * (size_t)c->n;
     const int64_t n = c->n;
-    j->nsteps = 0;
     j->every = every;
     // Checkpoints every `every` steps; the last `nkeep` segments also keep
     // their per-step states (U, U*, p*) from this forward, as many as the
@@ -967,8 +967,10 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
 extern "C" int bx_adjoint_backward(bx_ctx *c, const double *lam_T, double *lam_0)
 {
     if (!c || !lam_T || !lam_0) return set_error(BX_ERR_ARG, "bx_adjoint_backward: null argument");
-    if (!c->adj || c->adj->every <= 0 || !c->adj->ckpt || c->adj->stores.empty())
-        return set_error(BX_ERR_STATE, "bx_adjoint_backward: no checkpointed forward (bx_adjoint_forward with every > 0)");
+    if (!c->adj || c->adj->every <= 0 || !c->adj->ckpt || c->adj->stores.empty() || c->adj->nsteps <= 0)
+        return set_error(BX_ERR_STATE,
+                         "bx_adjoint_backward: no completed checkpointed forward (bx_adjoint_forward with nsteps > 0, "
+                         "every > 0)");
     BX_HIP(hipSetDevice(c->device));
     bx_adj *j = c->adj;
     const int64_t n = c->n;

@@ -262,7 +262,9 @@ int bx_boxes_dts(bx_boxes *b, double *dts);
  * bx_adjoint_backward: lam_T and lam_0 are DEVICE pointers to 5 SoA planes of
  * ngrid doubles (rho, mx, my, mz, E): the adjoint of the final state of the
  * last checkpointed forward, and the resulting adjoint of its initial state
- * (ghost-cell adjoints folded into their donors).  The forward keeps the
+ * (ghost-cell adjoints folded into their donors).  A forward that was refused
+ * or failed, took no steps or kept no checkpoints (every == 0) leaves nothing
+ * to sweep: the backward then returns BX_ERR_STATE.  The forward keeps the
  * per-step states of its last segments in HBM, as many as the store budget
  * holds (bx_adjoint_store; at least one); every other segment is recomputed
  * from its checkpoint before its reverse sweep.  Both give the same bits.

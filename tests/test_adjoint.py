@@ -242,3 +242,41 @@ def test_adjoint_full_size_matches_central_differences():
     print(f"smooth entries: {len(errs)}, worst err/max|g| {max(errs):.2e}, median {np.median(errs):.2e}; "
           f"kinks {kinks}", flush=True)
     assert len(kinks) <= 4
+
+
+@pytest.mark.gpu
+def test_adjoint_refused_forward_leaves_nothing_to_sweep():
+    """The backward needs a completed checkpointed forward: before any
+    forward, after one that keeps no checkpoints, and after one refused for a
+    state that is not regular (a density below 1e-4), it returns
+    BX_ERR_STATE instead of sweeping an earlier run."""
+    import torch
+
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.adjoint import AdjointStepper
+
+    prm = box_params(12)
+    ic = ics.box_ic(prm, seed=4)
+    dev = torch.device("cuda", 0)
+    rho0 = torch.as_tensor(ic["cons"][:, 0].copy(), device=dev)
+    p0 = torch.as_tensor(ic["p"].copy(), device=dev)
+    g = torch.ones_like(rho0)
+
+    def refused(call, code):
+        with pytest.raises(nv.BoxzyError) as e:
+            call()
+        assert f"({code})" in str(e.value)
+
+    st = AdjointStepper(prm, ic["u"], 4, 2)
+    refused(lambda: st.backward(g), nv.BX_ERR_STATE)
+    st.forward(rho0, p0)
+    g_ok = st.backward(g)
+    st.forward(rho0, p0, checkpoint=False)
+    refused(lambda: st.backward(g), nv.BX_ERR_STATE)
+    st.forward(rho0, p0)
+    assert torch.equal(st.backward(g)[0], g_ok[0])
+    low = rho0.clone()
+    interior = np.flatnonzero(ics.boundary_flags(prm).reshape(-1) == 0)
+    low[int(interior[len(interior) // 2])] = 5e-5
+    refused(lambda: st.forward(low, p0), nv.BX_ERR_UNSUPPORTED)
+    refused(lambda: st.backward(g), nv.BX_ERR_STATE)
