@@ -73,6 +73,11 @@ def grid_of_row(prm: HydroParams, row: Rows) -> Dict[str, np.ndarray]:
 
 
 def _concat(tubes: Sequence[Tuple[HydroParams, Rows]]):
+    for i, (prm, r) in enumerate(tubes):
+        for key, k in (("cons", 5), ("u", 3), ("p", 1)):
+            if np.asarray(r[key]).size != prm.nx * k:
+                raise ValueError(f"tube {i}: {key} has {np.asarray(r[key]).size} values, expected {prm.nx * k} "
+                                 f"(one x-row)")
     cons = np.ascontiguousarray(np.concatenate([r["cons"] for _, r in tubes]), np.float64)
     u = np.ascontiguousarray(np.concatenate([r["u"] for _, r in tubes]), np.float64)
     p = np.ascontiguousarray(np.concatenate([r["p"] for _, r in tubes]), np.float64)
@@ -99,7 +104,9 @@ class TubeBatch:
         self.off = np.concatenate([[0], np.cumsum(self.nx)])
         if endtimes is None:
             endtimes = [prm.endtime for prm, _ in self.tubes]
-        self.endtimes = np.ascontiguousarray(endtimes, np.float64)
+        self.endtimes = np.ascontiguousarray(endtimes, np.float64).reshape(-1)
+        if len(self.endtimes) != self.n:
+            raise ValueError(f"{len(self.endtimes)} endtimes for {self.n} tubes")
         arr, cons, u, p = _concat(self.tubes)
         self._h = ctypes.c_void_p()
         nv.check(nv.lib().bx_batch_create(self.n, arr, nv.dptr(self.endtimes), nv.dptr(cons), nv.dptr(u),

@@ -44,13 +44,17 @@ class GridBatch:
         self.off = np.concatenate([[0], np.cumsum(self.ngrid)])
         if endtimes is None:
             endtimes = [prm.endtime for prm, _ in self.grids]
-        self.endtimes = np.ascontiguousarray(endtimes, np.float64)
+        self.endtimes = np.ascontiguousarray(endtimes, np.float64).reshape(-1)
+        if len(self.endtimes) != self.n:
+            raise ValueError(f"{len(self.endtimes)} endtimes for {self.n} grids")
+        for i, (prm, g) in enumerate(self.grids):
+            for key, k in (("cons", 5), ("u", 3), ("p", 1)):
+                if np.asarray(g[key]).size != prm.ngrid * k:
+                    raise ValueError(f"grid {i}: {key} has {np.asarray(g[key]).size} values, expected {prm.ngrid * k}")
         cons = np.ascontiguousarray(np.concatenate([np.asarray(g["cons"], np.float64).reshape(-1, 5)
                                                     for _, g in self.grids]))
         u = np.ascontiguousarray(np.concatenate([np.asarray(g["u"], np.float64).reshape(-1, 3) for _, g in self.grids]))
         p = np.ascontiguousarray(np.concatenate([np.asarray(g["p"], np.float64).reshape(-1) for _, g in self.grids]))
-        if len(cons) != self.off[-1] or len(u) != self.off[-1] or len(p) != self.off[-1]:
-            raise ValueError("a set-up's arrays do not match its grid size")
         arr = (nv.BxParams * self.n)(*[make_params(prm) for prm, _ in self.grids])
         self._h = ctypes.c_void_p()
         nv.check(nv.lib().bx_boxes_create(self.n, arr, nv.dptr(self.endtimes), nv.dptr(cons), nv.dptr(u),

@@ -91,6 +91,22 @@ def _batch(tubes, **kw):
     return TubeBatch(tubes, **kw)
 
 
+def test_batch_checks_rows_and_endtimes():
+    """Each tube's arrays must be one x-row of its grid, and there must be one
+    endtime per tube: refused on the host before any call into the library."""
+    from boxzyhydro_amd.batch import TubeBatch, check_tubes
+
+    tubes = ics.riemann_tubes(2, seed=3, lmin=64, lmax=96)
+    (pa, ra), (pb, rb) = tubes
+    short = {k: v[:-1] for k, v in ra.items()}
+    with pytest.raises(ValueError, match="tube 0"):
+        check_tubes([(pa, short), (pb, rb)])
+    with pytest.raises(ValueError, match="tube 0"):
+        TubeBatch([(pa, short), (pb, rb)])
+    with pytest.raises(ValueError, match="endtimes"):
+        TubeBatch(tubes, endtimes=[0.1, 0.1, 0.1])
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_matches_reference_goldens(golden, graph):

@@ -260,3 +260,22 @@ def test_batch_rejects_nonzero_starttime():
     prm = HydroParams(nx=8, ny=8, nz=8, starttime=1.0)
     with pytest.raises(ValueError):
         GridBatch([(prm, {"cons": np.ones((prm.ngrid, 5)), "u": np.zeros((prm.ngrid, 3)), "p": np.ones(prm.ngrid)})])
+
+
+def test_batch_checks_each_setup_and_the_endtimes():
+    """Sizes are checked per grid (two mismatched set-ups whose total is right
+    are refused) and there must be one endtime per grid; both before any call
+    into the library."""
+    from boxzyhydro_amd.boxes import GridBatch
+    from boxzyhydro_amd.params import HydroParams
+
+    a, b = HydroParams(nx=8, ny=8, nz=8), HydroParams(nx=9, ny=8, nz=8)
+
+    def setup(prm, extra=0):
+        n = prm.ngrid + extra
+        return {"cons": np.ones((n, 5)), "u": np.zeros((n, 3)), "p": np.ones(n)}
+
+    with pytest.raises(ValueError, match="grid 0"):
+        GridBatch([(a, setup(a, -8)), (b, setup(b, 8))])
+    with pytest.raises(ValueError, match="endtimes"):
+        GridBatch([(a, setup(a)), (b, setup(b))], endtimes=[0.1])
