@@ -315,7 +315,7 @@ int bx_destroy(bx_ctx *c)
     drop_step_graphs(c);
     void *dev[] = {c->cons, c->cons_old, c->u, c->p, c->adindx, c->qq, c->states, c->ghost_dst, c->ghost_src,
                    c->ghost_sign, c->ghost_scratch_src, c->ghost_scratch, c->ustar, c->pstar, c->unext, c->dsc,
-                   c->cons32, c->ustar32, c->unext32, c->pstar32, c->eflux, c->chunk0, c->chunk_dts};
+                   c->cons32, c->ustar32, c->unext32, c->pstar32, c->eflux, c->chunk0, c->chunk_dts, c->stage};
     for (void *p : dev)
         if (p) (void)hipFree(p);
     if (c->hsc) (void)hipHostFree(c->hsc);
@@ -361,26 +361,81 @@ int bx_sync(bx_ctx *c)
     return BX_OK;
 }
 
-// AoS (k, n) host array <-> k SoA device planes
+// Host arrays in the reference's layout, cons(5,n) etc. (field fastest), and
+// the context's planes (one field after another): the transfer moves the
+// caller's bytes as they are through a device staging buffer, and a kernel
+// reorders them on the device.  Element e of a chunk of cells [n0, n0+cnt/k)
+// is field e%k of cell n0+e/k.
+__global__ __launch_bounds__(256) void k_rows_to_planes(int64_t n0, int64_t cnt, int k, int64_t n,
+                                                        const double *__restrict__ rows, double *__restrict__ planes)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= cnt) return;
+    const int64_t cell = e / k;
+    planes[(int64_t)(e - cell * k) * n + n0 + cell] = rows[e];
+}
+
+__global__ __launch_bounds__(256) void k_planes_to_rows(int64_t n0, int64_t cnt, int k, int64_t n,
+                                                        const double *__restrict__ planes, double *__restrict__ rows)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= cnt) return;
+    const int64_t cell = e / k;
+    rows[e] = planes[(int64_t)(e - cell * k) * n + n0 + cell];
+}
+
+constexpr int64_t kStageDoubles = int64_t(1) << 23;  // 64 MiB
+
+static int ensure_stage(bx_ctx *c)
+{
+    if (c->stage) return BX_OK;
+    const int64_t len = std::min<int64_t>(kStageDoubles, 5 * c->n);  // at least one cell of any k <= 5
+    BX_HIP(hipMalloc((void **)&c->stage, sizeof(double) * (size_t)len));
+    c->stage_len = len;
+    return BX_OK;
+}
+
+// host array (k, n) -> k device planes
 static int upload_aos(bx_ctx *c, const double *h, int k, double *d)
 {
     const int64_t n = c->n;
-    std::vector<double> tmp((size_t)n * k);
-    for (int64_t i = 0; i < n; i++)
-        for (int f = 0; f < k; f++) tmp[(size_t)f * n + i] = h[(size_t)i * k + f];
-    BX_HIP(hipMemcpyAsync(d, tmp.data(), sizeof(double) * tmp.size(), hipMemcpyHostToDevice, c->stream));
+    if (k == 1) {
+        BX_HIP(hipMemcpyAsync(d, h, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+        BX_HIP(hipStreamSynchronize(c->stream));
+        return BX_OK;
+    }
+    if (int rc = ensure_stage(c)) return rc;
+    for (int64_t n0 = 0; n0 < n;) {
+        const int64_t m = std::min<int64_t>(n - n0, c->stage_len / k), cnt = m * k;
+        BX_HIP(hipMemcpyAsync(c->stage, h + n0 * k, sizeof(double) * (size_t)cnt, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_rows_to_planes, dim3(blocks_for(cnt, 256)), dim3(256), 0, c->stream, n0, cnt, k, n,
+                           c->stage, d);
+        BX_HIP(hipGetLastError());
+        n0 += m;
+    }
     BX_HIP(hipStreamSynchronize(c->stream));
     return BX_OK;
 }
 
+// k device planes -> host array (k, n)
 static int download_aos(bx_ctx *c, const double *d, int k, double *h)
 {
     const int64_t n = c->n;
-    std::vector<double> tmp((size_t)n * k);
-    BX_HIP(hipMemcpyAsync(tmp.data(), d, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost, c->stream));
+    if (k == 1) {
+        BX_HIP(hipMemcpyAsync(h, d, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+        BX_HIP(hipStreamSynchronize(c->stream));
+        return BX_OK;
+    }
+    if (int rc = ensure_stage(c)) return rc;
+    for (int64_t n0 = 0; n0 < n;) {
+        const int64_t m = std::min<int64_t>(n - n0, c->stage_len / k), cnt = m * k;
+        hipLaunchKernelGGL(k_planes_to_rows, dim3(blocks_for(cnt, 256)), dim3(256), 0, c->stream, n0, cnt, k, n, d,
+                           c->stage);
+        BX_HIP(hipGetLastError());
+        BX_HIP(hipMemcpyAsync(h + n0 * k, c->stage, sizeof(double) * (size_t)cnt, hipMemcpyDeviceToHost, c->stream));
+        n0 += m;
+    }
     BX_HIP(hipStreamSynchronize(c->stream));
-    for (int64_t i = 0; i < n; i++)
-        for (int f = 0; f < k; f++) h[(size_t)i * k + f] = tmp[(size_t)f * n + i];
     return BX_OK;
 }
 

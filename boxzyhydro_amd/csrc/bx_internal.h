@@ -95,6 +95,8 @@ struct bx_ctx {
     void *chunk0;          // 5 planes: the state at the start of a replayed chunk (rollback)
     int64_t graph_steps;   // steps taken from replays (kept, not undone)
     double *chunk_dts;     // dt of each step of a chunk
+    double *stage;         // device staging of host transfers (Fortran layout), stage_len doubles
+    int64_t stage_len;
 
     bx::DevScalars *dsc;   // device scalars
     bx::DevScalars *hsc;   // pinned host mirror
