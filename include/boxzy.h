@@ -72,7 +72,9 @@ int bx_ghost_table(const bx_params *prm, int64_t *nghost, int64_t *nscratch, int
 
 /* Host <-> device copies of the reference's global arrays (Fortran layout).
  * Any pointer may be NULL to skip that array.  upload also sets
- * cons_old = cons, as init_conditions.f90:69-77 does. */
+ * cons_old = cons, as init_conditions.f90:69-77 does.  The first transfer of
+ * a multi-field array allocates a device staging buffer of at most 64 MiB,
+ * which the context keeps until bx_destroy. */
 int bx_upload(bx_ctx *ctx, const double *cons, const double *u, const double *p, const double *adindx);
 int bx_download(bx_ctx *ctx, double *cons, double *u, double *p, double *adindx);
 int bx_download_cons_old(bx_ctx *ctx, double *cons_old);
