@@ -137,7 +137,7 @@ StageArgs base_args(bx_ctx *c, double dt)
 }
 
 template <int STAGE, typename S>
-static int launch_stage(bx_ctx *c, const StageArgs &a, hipEvent_t mid)
+static int launch_stage(bx_ctx *c, const StageArgs &a, hipEvent_t mid, S *eflux)
 {
     const int nzi = a.zhi - a.zlo;
     const int ly = c->fused_ly;
@@ -145,7 +145,7 @@ static int launch_stage(bx_ctx *c, const StageArgs &a, hipEvent_t mid)
         const unsigned gx = (unsigned)((c->nx - 4 + 63) / 64);
         const unsigned gy = (unsigned)((nzi + 3) / 4);
         const unsigned gz = (unsigned)((c->ny - 4 + ly - 1) / ly);
-        hipLaunchKernelGGL((k_ydir<STAGE, S>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, (S *)c->eflux, ly);
+        hipLaunchKernelGGL((k_ydir<STAGE, S>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, eflux, ly);
         BX_HIP(hipGetLastError());
     }
     if (mid) BX_HIP(hipEventRecord(mid, c->stream));
@@ -154,7 +154,7 @@ static int launch_stage(bx_ctx *c, const StageArgs &a, hipEvent_t mid)
         const unsigned gx = (unsigned)((nwaves + 3) / 4);
         const unsigned gy = 1;
         const unsigned gz = (unsigned)((nzi + a.lz - 1) / a.lz);
-        const S *ey = (const S *)c->eflux;
+        const S *ey = eflux;
         if (c->fused_occ == 4)
             hipLaunchKernelGGL((k_xz<STAGE, S, 4, false>), dim3(gx, gy, gz), dim3(256), 0, c->stream, a, ey);
         else if (c->fused_occ == 1)
@@ -174,8 +174,8 @@ int launch_materialize(bx_ctx *c)
     return BX_OK;
 }
 
-// Ghost table shallow, allocation of the fused buffers.
-int fused_prepare(bx_ctx *c)
+// Whether the fused kernels compute this context's step (no allocation).
+int fused_supported(const bx_ctx *c)
 {
     if (c->nscratch > 0)
         return set_error(BX_ERR_UNSUPPORTED, "fused step needs every ghost donor to be an interior cell");
@@ -184,24 +184,15 @@ int fused_prepare(bx_ctx *c)
     if (c->prm.fluxangmom) return set_error(BX_ERR_UNSUPPORTED, "fused step fluxes linear momentum (fluxangmom off)");
     if (c->grav_flags) return set_error(BX_ERR_UNSUPPORTED, "fused step assumes the zero-potential force");
     if (c->eos) return set_error(BX_ERR_UNSUPPORTED, "fused step assumes a fixed gamma (H2STAT=-1)");
-    if (c->prm.precision == BX_PREC_MIXED) {
-        const size_t pf = sizeof(float) * (size_t)c->n;
-        if (!c->cons32) BX_HIP(hipMalloc((void **)&c->cons32, 5 * pf));
-        if (!c->ustar32) BX_HIP(hipMalloc((void **)&c->ustar32, 5 * pf));
-        if (!c->unext32) BX_HIP(hipMalloc((void **)&c->unext32, 5 * pf));
-        if (!c->pstar32) BX_HIP(hipMalloc((void **)&c->pstar32, pf));
-    }
-    const size_t pb = sizeof(double) * (size_t)c->n;
-    if (!c->eflux) BX_HIP(hipMalloc((void **)&c->eflux, 5 * pb));
-    if (!c->ustar) BX_HIP(hipMalloc((void **)&c->ustar, 5 * pb));
-    if (!c->unext) BX_HIP(hipMalloc((void **)&c->unext, 5 * pb));
-    if (!c->pstar) BX_HIP(hipMalloc((void **)&c->pstar, pb));
-    // defaults from the autotuner on the 256^3 box (FP64: 64 rows, 32 planes;
-    // mixed: 64 rows, 16 planes, y-term loaded early)
-    // default work split (autotuned on the 256^3 box: FP64 64 rows and 32 planes
-    // per wave, mixed 16 planes with the y-term loaded early), shortened on
-    // smaller grids until the y kernel has about 2 and the x/z kernel about 4
-    // waves per SIMD to run
+    return BX_OK;
+}
+
+// Default work split of the stage kernels (autotuned on the 256^3 box: FP64
+// 64 rows and 32 planes per wave, mixed 16 planes with the y-term loaded
+// early), shortened on smaller grids until the y kernel has about 2 and the
+// x/z kernel about 4 waves per SIMD to run.  Allocates nothing.
+void fused_defaults(bx_ctx *c)
+{
     const int64_t want = 2 * 1024;
     const int nzi = c->flatz ? c->nz : c->nz - 4;
     if (c->fused_ly == 0) {
@@ -216,6 +207,25 @@ int fused_prepare(bx_ctx *c)
         c->fused_lz = lz;
         c->fused_occ = c->prm.precision == BX_PREC_MIXED ? 1 : 0;
     }
+}
+
+// Ghost table shallow, allocation of the fused buffers.
+int fused_prepare(bx_ctx *c)
+{
+    if (int rc = fused_supported(c)) return rc;
+    if (c->prm.precision == BX_PREC_MIXED) {
+        const size_t pf = sizeof(float) * (size_t)c->n;
+        if (!c->cons32) BX_HIP(hipMalloc((void **)&c->cons32, 5 * pf));
+        if (!c->ustar32) BX_HIP(hipMalloc((void **)&c->ustar32, 5 * pf));
+        if (!c->unext32) BX_HIP(hipMalloc((void **)&c->unext32, 5 * pf));
+        if (!c->pstar32) BX_HIP(hipMalloc((void **)&c->pstar32, pf));
+    }
+    const size_t pb = sizeof(double) * (size_t)c->n;
+    if (!c->eflux) BX_HIP(hipMalloc((void **)&c->eflux, 5 * pb));
+    if (!c->ustar) BX_HIP(hipMalloc((void **)&c->ustar, 5 * pb));
+    if (!c->unext) BX_HIP(hipMalloc((void **)&c->unext, 5 * pb));
+    if (!c->pstar) BX_HIP(hipMalloc((void **)&c->pstar, pb));
+    fused_defaults(c);
     return BX_OK;
 }
 
@@ -275,7 +285,7 @@ static int fused_step_t(bx_ctx *c, double dt, S *u0, S *us, S *ps, S *un, bool r
     a.uout = us;
     a.pout = ps;
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[0], c->stream));
-    if ((rc = launch_stage<1, S>(c, a, c->prof_on ? c->ev[1] : nullptr))) return rc;
+    if ((rc = launch_stage<1, S>(c, a, c->prof_on ? c->ev[1] : nullptr, (S *)c->eflux))) return rc;
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[2], c->stream));
     if ((rc = ghosts6<S>(c, us, ps))) return rc;
     StageArgs b = base_args(c, dt);
@@ -286,7 +296,7 @@ static int fused_step_t(bx_ctx *c, double dt, S *u0, S *us, S *ps, S *un, bool r
     b.uout = un;
     b.rate = &c->dsc->rate_bits[0];
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[3], c->stream));
-    if ((rc = launch_stage<2, S>(c, b, c->prof_on ? c->ev[4] : nullptr))) return rc;
+    if ((rc = launch_stage<2, S>(c, b, c->prof_on ? c->ev[4] : nullptr, (S *)c->eflux))) return rc;
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[5], c->stream));
     if ((rc = ghosts6<S>(c, un, (S *)nullptr))) return rc;
     return BX_OK;
@@ -342,7 +352,7 @@ static int enqueue_pair(bx_ctx *c, S *u0, S *us, S *ps, S *un)
         a.uin = src;
         a.uout = us;
         a.pout = ps;
-        if ((rc = launch_stage<1, S>(c, a, nullptr))) return rc;
+        if ((rc = launch_stage<1, S>(c, a, nullptr, (S *)c->eflux))) return rc;
         if ((rc = ghosts6<S>(c, us, ps))) return rc;
         StageArgs b = base_args(c, 0.0);
         b.dtp = &c->dsc->dt;
@@ -351,7 +361,7 @@ static int enqueue_pair(bx_ctx *c, S *u0, S *us, S *ps, S *un)
         b.uold = src;
         b.uout = dst;
         b.rate = &c->dsc->rate_bits[0];
-        if ((rc = launch_stage<2, S>(c, b, nullptr))) return rc;
+        if ((rc = launch_stage<2, S>(c, b, nullptr, (S *)c->eflux))) return rc;
         if ((rc = ghosts6<S>(c, dst, (S *)nullptr))) return rc;
     }
     return BX_OK;
@@ -421,6 +431,41 @@ int replay_step_pairs(bx_ctx *c, int pairs)
 int fused_step_into(bx_ctx *c, double dt, double *u0, double *us, double *ps, double *un, bool reset_flags)
 {
     return fused_step_t<double>(c, dt, u0, us, ps, un, reset_flags);
+}
+
+__global__ void k_clear_scalars(unsigned int *flags, unsigned long long *rate)
+{
+    *flags = 0u;
+    *rate = 0ull;
+}
+
+// One FP64 fused step u0 -> us, ps -> un (ey: the y-term planes) on the
+// context's stream with the flags and the Courant rate accumulated into the
+// given device words, which
+// a kernel clears first (no memset nodes, so the sequence can be captured).
+// Allocates nothing; the tile defaults must be set (fused_defaults).
+int fused_step_words(bx_ctx *c, double dt, const double *u0, double *us, double *ps, double *un, double *ey,
+                     unsigned int *flags, unsigned long long *rate)
+{
+    int rc;
+    hipLaunchKernelGGL(k_clear_scalars, dim3(1), dim3(1), 0, c->stream, flags, rate);
+    BX_HIP(hipGetLastError());
+    StageArgs a = base_args(c, dt);
+    a.flags = flags;
+    a.uin = u0;
+    a.uout = us;
+    a.pout = ps;
+    if ((rc = launch_stage<1, double>(c, a, nullptr, ey))) return rc;
+    if ((rc = ghosts6<double>(c, us, ps))) return rc;
+    StageArgs b = base_args(c, dt);
+    b.flags = flags;
+    b.uin = us;
+    b.pin = ps;
+    b.uold = u0;
+    b.uout = un;
+    b.rate = rate;
+    if ((rc = launch_stage<2, double>(c, b, nullptr, ey))) return rc;
+    return ghosts6<double>(c, un, (double *)nullptr);
 }
 
 // A step whose dt is formed on the device from the Courant rate of its state

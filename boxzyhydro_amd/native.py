@@ -29,6 +29,8 @@ BX_STEP_GRAPH = 2
 BX_GRAV_POTENTIAL = 1
 BX_GRAV_EXTERNAL = 2
 BX_GRAV_COM_SEQUENTIAL = 4
+BX_STATIC_MAX_CELLS = 1048576
+BX_STATIC_BYTES_PER_CELL = 88
 
 
 class BxParams(ctypes.Structure):
@@ -93,6 +95,7 @@ _SIGS = {
     "bx_avisc": [_VP, _DP],
     "bx_step": [_VP, _I32, _I32, ctypes.c_double, _DP, _DP, ctypes.POINTER(_I32), _DP,
                 ctypes.POINTER(_I32)],
+    "bx_step_static": [_VP, _VP, _VP, ctypes.c_double, _VP, _VP, _VP],
     "bx_fallback_count": [_VP, ctypes.POINTER(ctypes.c_int64)],
     "bx_graph_step_count": [_VP, ctypes.POINTER(ctypes.c_int64)],
     "bx_tune": [_VP, _I32, _I32, _I32],

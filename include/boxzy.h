@@ -112,6 +112,33 @@ int bx_avisc(bx_ctx *ctx, double *qq);            /* avisc.f90:4-26; qq(3,ngrid)
 int bx_step(bx_ctx *ctx, int32_t mode, int32_t nsteps, double endtime, double *time, double *dtold,
             int32_t *step, double *dts, int32_t *taken);
 
+/* One fused FP64 step that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It computes what bx_step(BX_STEP_FUSED) computes for one step of full dt
+ * `dt` from a regular state (ghost cells filled with their donors' values,
+ * u = m/rho, rho >= 1e-4, |u| <= vlimit), bit for bit, from the caller's
+ * DEVICE planes cons_in (5 SoA planes of ngrid doubles: rho, mx, my, mz, E)
+ * into cons_out (5 planes, ghosts filled; must not alias cons_in).  The
+ * context supplies the grid, the parameters and the ghost table; its own
+ * state is not read or written.  U*, p* and the other intermediates live in
+ * a static device array of BX_STATIC_MAX_CELLS * BX_STATIC_BYTES_PER_CELL
+ * bytes (plus 256) that each GPU holds until the process ends: ngrid above
+ * BX_STATIC_MAX_CELLS is refused (BX_ERR_ARG).  Every call on one GPU shares
+ * that array, so calls that overlap on one GPU (other streams, other
+ * threads) race.  rate_out and flags_out are DEVICE words (either may be
+ * NULL): the Courant max-rate of cons_out (the next dt is cfl/rate,
+ * main.f90:266-281) and the irregular branches met (bit 1: rho < 1e-4,
+ * bit 2: the vlimit clamp; the step is then not the reference's and must be
+ * redone with bx_step; bit 8: a floor acted, informational).  Work is
+ * enqueued on `stream` (a hipStream_t; NULL: the context's stream) and not
+ * waited for.  FP64 contexts whose fused step is supported (every ghost
+ * donor an interior cell, fixed gamma, plain flux, zero potential). */
+#define BX_STATIC_MAX_CELLS 1048576
+#define BX_STATIC_BYTES_PER_CELL 88
+int bx_step_static(bx_ctx *ctx, const double *cons_in, double *cons_out, double dt, double *rate_out,
+                   uint32_t *flags_out, void *stream);
+
 /* Number of steps that bx_step(BX_STEP_FUSED) had to redo with the exact
  * sequence since the context was created. */
 int bx_fallback_count(const bx_ctx *ctx, int64_t *count);
