@@ -4,8 +4,11 @@ step (bx_adjoint_* of include/boxzy.h) behind a ``torch.autograd.Function``.
 The differentiated map takes the initial density and pressure (velocity held
 fixed, energy rebuilt as in the set-up, E = p/(gamma-1) + rho |u|^2/2) through
 ``nsteps`` steps of main.f90:211-555 with a frozen dt sequence to the final
-state.  The forward keeps a checkpoint every ``every`` steps; the backward
-recomputes each segment from its checkpoint and sweeps it in reverse.
+state.  The forward keeps a checkpoint every ``every`` steps and, for as many
+of the last segments as its store budget holds (``AdjointStepper.store``;
+default half the free device memory), the per-step states of the segment.
+The backward sweeps the segments in reverse: kept segments as they are, the
+others recomputed from their checkpoints first.  Both give the same bits.
 
 torch is used for the tensors and the autograd plumbing only: the steps and
 their adjoint run in libboxzy_hip.so on the context's own stream.
@@ -48,6 +51,9 @@ class AdjointStepper:
         self.uu = self.u[0] * self.u[0] + self.u[1] * self.u[1] + self.u[2] * self.u[2]
         self.gm1 = prm.gammafix - 1.0
         self.dts: Optional[np.ndarray] = None
+        # checkpointed forwards run so far; the context holds the checkpoints
+        # of the last one only
+        self.generation = 0
 
     # --------------------------------------------------------------- helpers
     def cons_of(self, rho, p):
@@ -81,6 +87,8 @@ class AdjointStepper:
         self._set(self.cons_of(rho, p).contiguous())
         out = np.zeros(self.nsteps)
         dptr = nv.dptr(np.ascontiguousarray(dts, np.float64)) if dts is not None else nv.dptr(None)
+        if checkpoint:
+            self.generation += 1
         nv.check(nv.lib().bx_adjoint_forward(self.grid._h, self.nsteps, self.every if checkpoint else 0, dptr,
                                              nv.dptr(out)), "bx_adjoint_forward")
         if dts is None:
@@ -129,11 +137,23 @@ def hydro_function():
         @staticmethod
         def forward(ctx, rho, p, stepper: AdjointStepper):
             ctx.stepper = stepper
-            return stepper.forward(rho.detach(), p.detach(), dts=stepper.dts, checkpoint=True)
+            out = stepper.forward(rho.detach(), p.detach(), dts=stepper.dts, checkpoint=True)
+            # the context keeps the checkpoints of its last forward only: a
+            # later forward on the same stepper (another rollout in the same
+            # graph) replaces them, and this node's backward then runs its
+            # own forward again from the saved inputs
+            ctx.generation = stepper.generation
+            ctx.dts = np.array(stepper.dts, copy=True)
+            ctx.save_for_backward(rho.detach(), p.detach())
+            return out
 
         @staticmethod
         def backward(ctx, grad):
-            g_rho, g_p = ctx.stepper.backward(grad.contiguous())
+            st = ctx.stepper
+            if st.generation != ctx.generation:
+                rho, p = ctx.saved_tensors
+                st.forward(rho, p, dts=ctx.dts, checkpoint=True)
+            g_rho, g_p = st.backward(grad.contiguous())
             return g_rho, g_p, None
 
     return HydroSteps

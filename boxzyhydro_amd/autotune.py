@@ -1,7 +1,8 @@
 """Autotuner for the fused step's work split.
 
 Times a few steps of each candidate (y rows per wave of the y-direction
-kernel, z planes per wave of the x/z kernel, variant of the x/z kernel) on the grid it is given and keeps the fastest.  Results of the step do
+kernel, z planes per wave of the x/z kernel or of the tiled stage kernel, the
+stage variant) on the grid it is given and keeps the fastest.  Results of the step do
 not depend on the choice, so tuning can run on the production grid itself; the
 state it leaves behind is stepped forward by the timing steps, so tune on a
 copy (or re-upload the initial condition afterwards).
@@ -14,7 +15,10 @@ from typing import Iterable, List, Tuple
 
 from . import native as nv
 
-DEFAULT_CANDIDATES = list(itertools.product((16, 32, 64), (16, 32, 64), (0, 1, 4)))
+# (y rows, z planes, variant): the two-kernel stages (variants 0, 1, 4) and
+# the one-kernel tiled stages (variants 10, 11; y rows unused)
+DEFAULT_CANDIDATES = (list(itertools.product((16, 32, 64), (16, 32, 64), (0, 1, 4))) +
+                      list(itertools.product((64,), (16, 32, 64), (10, 11))))
 
 
 def tune_step(grid, candidates: Iterable[Tuple[int, int, int]] = DEFAULT_CANDIDATES, steps: int = 3,

@@ -681,8 +681,12 @@ static int adj_init(bx_ctx *c)
     std::vector<int32_t> dst(ng), src(ng), sgn(ng);
     if (ng) {
         BX_HIP(hipMemcpy(dst.data(), c->ghost_dst, sizeof(int32_t) * ng, hipMemcpyDeviceToHost));
-        BX_HIP(hipMemcpy(src.data(), c->ghost_src, sizeof(int32_t) * ng, hipMemcpyDeviceToHost));
-        BX_HIP(hipMemcpy(sgn.data(), c->ghost_sign, sizeof(int32_t) * ng, hipMemcpyDeviceToHost));
+        // the table of the fused step (a pair of set_ghost_cells calls composed)
+        const int32_t *ts = c->nscratch ? c->fghost_src : c->ghost_src;
+        const int32_t *tg = c->nscratch ? c->fghost_sign : c->ghost_sign;
+        if (!ts || !tg) return set_error(BX_ERR_UNSUPPORTED, "adjoint: ghost donors that are ghosts are not supported");
+        BX_HIP(hipMemcpy(src.data(), ts, sizeof(int32_t) * ng, hipMemcpyDeviceToHost));
+        BX_HIP(hipMemcpy(sgn.data(), tg, sizeof(int32_t) * ng, hipMemcpyDeviceToHost));
     }
     std::vector<int64_t> order(ng);
     for (int64_t k = 0; k < ng; k++) {
@@ -919,7 +923,20 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
         h->logk = 0;
         BX_HIP(hipMemcpyAsync(c->dsc, h, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream));
     }
+    // Once the steps have started, the context's state, its u/p/cons_old planes
+    // and the Courant rate no longer go together: an error or a refusal marks
+    // them stale (the next bx_step checks the state again) and, when a
+    // checkpoint of the start was kept, puts the starting state back.
+    auto invalidate = [&](bool restore) {
+        c->fused_valid = false;
+        c->aux_stale = true;
+        c->old_stale = true;
+        if (restore && nseg > 0)
+            (void)hipMemcpyAsync(c->cons, j->ckpt, 5 * pb, hipMemcpyDeviceToDevice, c->stream);
+        (void)hipStreamSynchronize(c->stream);
+    };
     auto refuse = [&](unsigned f) -> int {
+        invalidate(true);
         if (f & FLAG_IRREGULAR) return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: the run met rho<1e-4 or |u|>vlimit");
         return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: the run met a floor of the update (not differentiated)");
     };
@@ -939,9 +956,13 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
         double *u0 = kept ? U(i, t) : c->cons, *us = kept ? Sx(i, t) : c->ustar;
         double *ps = kept ? Px(i, t) : c->pstar, *un = kept ? U(i, t + 1) : c->unext;
         if (frozen) {
-            if ((rc = fused_step_into(c, dts_in[k], u0, us, ps, un, false))) return rc;
+            if ((rc = fused_step_into(c, dts_in[k], u0, us, ps, un, false))) {
+                invalidate(false);
+                return rc;
+            }
             j->dts[k] = dts_in[k];
         } else if ((rc = fused_step_device_dt(c, u0, us, ps, un, j->dlog, nsteps))) {
+            invalidate(false);
             return rc;
         }
         if (!kept) std::swap(c->cons, c->unext);
@@ -952,7 +973,10 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
     if (nkeep > 0)  // the final state, from the end of the last kept segment
         BX_HIP(hipMemcpyAsync(c->cons, U(nkeep - 1, nsteps - (nseg - 1) * every), 5 * pb, hipMemcpyDeviceToDevice,
                               c->stream));
-    if ((rc = read_flags_rate(c, &flags, &rate))) return rc;  // also completes the dt read-back
+    if ((rc = read_flags_rate(c, &flags, &rate))) {  // also completes the dt read-back
+        invalidate(false);
+        return rc;
+    }
     if (flags & (FLAG_IRREGULAR | FLAG_FLOOR)) return refuse(flags);
     if (dts_out)
         for (int k = 0; k < nsteps; k++) dts_out[k] = j->dts[k];

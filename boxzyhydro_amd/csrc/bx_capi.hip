@@ -254,6 +254,26 @@ int bx_create(const bx_params *prm, int32_t device, bx_ctx **out)
     }
     c->nghost = (int64_t)gd.size();
     c->nscratch = (int64_t)gx.size();
+    // The pair of calls composed: a ghost whose donor is the pre-call value of
+    // ghost h gets h's value after the first call, which is h's donor when
+    // that is an interior cell (sign flips compose).  Otherwise the pair
+    // leaves a value from before it and the fused step cannot apply.
+    std::vector<int32_t> fs(gs), fg(gg);
+    c->fghost_ok = true;
+    if (!gx.empty()) {
+        std::vector<int64_t> entry((size_t)n, -1);
+        for (size_t k = 0; k < gd.size(); k++) entry[(size_t)gd[k]] = (int64_t)k;
+        for (size_t k = 0; k < gd.size() && c->fghost_ok; k++) {
+            if (gs[k] >= 0) continue;
+            const int64_t h = entry[(size_t)gx[(size_t)(-gs[k] - 1)]];
+            if (h < 0 || gs[(size_t)h] < 0) {
+                c->fghost_ok = false;
+                break;
+            }
+            fs[k] = gs[(size_t)h];
+            fg[k] = gg[k] ^ gg[(size_t)h];
+        }
+    }
 
     auto fail = [&](hipError_t e, const char *what) {
         int r = hip_check(e, what);
@@ -285,6 +305,17 @@ int bx_create(const bx_params *prm, int32_t device, bx_ctx **out)
         if ((e = hipMemcpy(c->ghost_sign, gg.data(), sizeof(int32_t) * gg.size(), hipMemcpyHostToDevice)) != hipSuccess)
             return fail(e, "hipMemcpy ghosts");
     }
+    if (!gx.empty() && c->fghost_ok) {
+        if ((e = hipMalloc((void **)&c->fghost_src, sizeof(int32_t) * ng)) != hipSuccess)
+            return fail(e, "hipMalloc ghosts");
+        if ((e = hipMalloc((void **)&c->fghost_sign, sizeof(int32_t) * ng)) != hipSuccess)
+            return fail(e, "hipMalloc ghosts");
+        if ((e = hipMemcpy(c->fghost_src, fs.data(), sizeof(int32_t) * fs.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "hipMemcpy ghosts");
+        if ((e = hipMemcpy(c->fghost_sign, fg.data(), sizeof(int32_t) * fg.size(), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+            return fail(e, "hipMemcpy ghosts");
+    }
     if (!gx.empty()) {
         if ((e = hipMalloc((void **)&c->ghost_scratch_src, sizeof(int32_t) * gx.size())) != hipSuccess)
             return fail(e, "hipMalloc ghost scratch");
@@ -314,7 +345,8 @@ int bx_destroy(bx_ctx *c)
     gravity_free(c);
     drop_step_graphs(c);
     void *dev[] = {c->cons, c->cons_old, c->u, c->p, c->adindx, c->qq, c->states, c->ghost_dst, c->ghost_src,
-                   c->ghost_sign, c->ghost_scratch_src, c->ghost_scratch, c->ustar, c->pstar, c->unext, c->dsc,
+                   c->ghost_sign, c->ghost_scratch_src, c->ghost_scratch, c->fghost_src, c->fghost_sign,
+                   c->ustar, c->pstar, c->unext, c->dsc,
                    c->cons32, c->ustar32, c->unext32, c->pstar32, c->eflux, c->chunk0, c->chunk_dts, c->stage};
     for (void *p : dev)
         if (p) (void)hipFree(p);
@@ -606,6 +638,14 @@ int bx_graph_step_count(const bx_ctx *c, int64_t *count)
     return BX_OK;
 }
 
+int bx_step_counts(const bx_ctx *c, int64_t *fused, int64_t *exact)
+{
+    if (!c) return set_error(BX_ERR_ARG, "bx_step_counts: null context");
+    if (fused) *fused = c->nfused;
+    if (exact) *exact = c->nexact;
+    return BX_OK;
+}
+
 }  // extern "C"
 
 namespace bx {
@@ -665,6 +705,7 @@ static int exact_step(bx_ctx *c, double endtime, double *time, double *dtold, in
     *time = *time + dt;
     *dt_out = dt;
     c->old_stale = false;
+    c->nexact++;
     return BX_OK;
 }
 
@@ -756,6 +797,7 @@ static int graph_chunk(bx_ctx *c, int K, double endtime, double *time, double *d
     c->aux_stale = true;
     c->old_stale = true;
     c->graph_steps += K;
+    c->nfused += K;
     *taken = K;
     return BX_OK;
 }
@@ -850,6 +892,7 @@ static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, d
         }
         if (c->state_in_32) std::swap(c->cons32, c->unext32);
         else std::swap(c->cons, c->unext);
+        c->nfused++;
         c->aux_stale = true;
         c->old_stale = true;
         *time = *time + dt;  // main.f90:544-545 (dt*half*two == dt)
@@ -905,7 +948,8 @@ extern "C" int bx_tune(bx_ctx *c, int32_t y_chunk, int32_t z_chunk, int32_t occu
 {
     if (!c) return set_error(BX_ERR_ARG, "bx_tune: null context");
     if (y_chunk < 1 || z_chunk < 1) return set_error(BX_ERR_ARG, "bx_tune: chunks must be >= 1");
-    if (occupancy != 0 && occupancy != 1 && occupancy != 4) return set_error(BX_ERR_ARG, "bx_tune: variant 0, 1 or 4");
+    if (occupancy != 0 && occupancy != 1 && occupancy != 4 && !(occupancy >= 10 && occupancy <= 11))
+        return set_error(BX_ERR_ARG, "bx_tune: variant 0, 1, 4, 10 or 11");
     c->fused_ly = y_chunk;
     c->fused_lz = z_chunk;
     c->fused_occ = occupancy;

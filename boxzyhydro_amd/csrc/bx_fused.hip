@@ -29,9 +29,19 @@
 //     order and applies the stage update.
 #include <cmath>
 
-#include "bx_waves.h"
+#include "bx_tile.h"
 
 namespace bx {
+
+// One stage as one kernel: x-y tile of 64 x TY outputs with its +-2 halo in
+// LDS, marching in z (bx_tile.h: tile_stage)
+template <int STAGE, typename S, int TY>
+__global__ __launch_bounds__(64 * (TY + 4)) void k_stage(StageArgs a0)
+{
+    __shared__ TileLds<S, TY> lds;
+    const StageArgs a = with_dt(a0);
+    tile_stage<STAGE, S, TY>(a, lds, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 
 // y-direction kernel: one launch over the interior (bx_waves.h: ydir_wave)
@@ -136,9 +146,28 @@ StageArgs base_args(bx_ctx *c, double dt)
     return a;
 }
 
+template <int STAGE, typename S, int TY>
+static int launch_tile(bx_ctx *c, const StageArgs &a)
+{
+    const int nzi = a.zhi - a.zlo;
+    const unsigned gx = (unsigned)((c->nx - 4 + 63) / 64);
+    const unsigned gy = (unsigned)((c->ny - 4 + TY - 1) / TY);
+    const unsigned gz = (unsigned)((nzi + a.lz - 1) / a.lz);
+    hipLaunchKernelGGL((k_stage<STAGE, S, TY>), dim3(gx, gy, gz), dim3(64 * (TY + 4)), 0, c->stream, a);
+    BX_HIP(hipGetLastError());
+    return BX_OK;
+}
+
 template <int STAGE, typename S>
 static int launch_stage(bx_ctx *c, const StageArgs &a, hipEvent_t mid, S *eflux)
 {
+    if (c->fused_occ >= 10) {
+        if (mid) BX_HIP(hipEventRecord(mid, c->stream));
+        switch (c->fused_occ) {
+        case 10: return launch_tile<STAGE, S, 4>(c, a);
+        default: return launch_tile<STAGE, S, 8>(c, a);
+        }
+    }
     const int nzi = a.zhi - a.zlo;
     const int ly = c->fused_ly;
     {
@@ -177,8 +206,9 @@ int launch_materialize(bx_ctx *c)
 // Whether the fused kernels compute this context's step (no allocation).
 int fused_supported(const bx_ctx *c)
 {
-    if (c->nscratch > 0)
-        return set_error(BX_ERR_UNSUPPORTED, "fused step needs every ghost donor to be an interior cell");
+    if (c->nscratch > 0 && !c->fghost_ok)
+        return set_error(BX_ERR_UNSUPPORTED, "fused step needs the ghost donors of a pair of set_ghost_cells calls "
+                                             "to be interior cells");
     if (c->grav[0] != 0.0 || c->grav[1] != 0.0 || c->grav[2] != 0.0)
         return set_error(BX_ERR_UNSUPPORTED, "fused step assumes zero gravity");
     if (c->prm.fluxangmom) return set_error(BX_ERR_UNSUPPORTED, "fused step fluxes linear momentum (fluxangmom off)");
@@ -187,9 +217,11 @@ int fused_supported(const bx_ctx *c)
     return BX_OK;
 }
 
-// Default work split of the stage kernels (autotuned on the 256^3 box: FP64
-// 64 rows and 32 planes per wave, mixed 16 planes with the y-term loaded
-// early), shortened on smaller grids until the y kernel has about 2 and the
+// Default work split of the stage kernels.  FP64: one kernel per stage
+// (64 x 8 tiles, variant 11) with z-chunks of 64 planes, shortened until the
+// grid has two workgroups per CU; the mixed form: the two-kernel stage (64
+// rows and 16 planes per wave, y-term loaded early; autotuned on the 256^3
+// box), shortened on smaller grids until the y kernel has about 2 and the
 // x/z kernel about 4 waves per SIMD to run.  Allocates nothing.
 void fused_defaults(bx_ctx *c)
 {
@@ -201,11 +233,19 @@ void fused_defaults(bx_ctx *c)
         c->fused_ly = ly;
     }
     if (c->fused_lz == 0) {
-        int lz = c->prm.precision == BX_PREC_MIXED ? 16 : 32;
-        const int64_t rowwaves = ((int64_t)(c->ny - 4) * c->nx + kSegOut - 1) / kSegOut;
-        while (lz > 4 && rowwaves * ((nzi + lz - 1) / lz) < 2 * want) lz /= 2;
-        c->fused_lz = lz;
-        c->fused_occ = c->prm.precision == BX_PREC_MIXED ? 1 : 0;
+        if (c->prm.precision == BX_PREC_MIXED) {
+            int lz = 16;
+            const int64_t rowwaves = ((int64_t)(c->ny - 4) * c->nx + kSegOut - 1) / kSegOut;
+            while (lz > 4 && rowwaves * ((nzi + lz - 1) / lz) < 2 * want) lz /= 2;
+            c->fused_lz = lz;
+            c->fused_occ = 1;
+        } else {
+            int lz = 64;
+            const int64_t tiles = (int64_t)((c->nx - 4 + 63) / 64) * ((c->ny - 4 + 7) / 8);
+            while (lz > 4 && tiles * ((nzi + lz - 1) / lz) < 512) lz /= 2;
+            c->fused_lz = lz;
+            c->fused_occ = 11;
+        }
     }
 }
 
@@ -247,7 +287,8 @@ static int ghosts6(bx_ctx *c, S *planes, S *pstar)
 {
     if (c->nghost == 0) return BX_OK;
     hipLaunchKernelGGL(k_ghosts6<S>, dim3(blocks_for(c->nghost, 256)), dim3(256), 0, c->stream, c->n, c->nghost,
-                       c->ghost_dst, c->ghost_src, c->ghost_sign, planes, pstar);
+                       c->ghost_dst, c->nscratch ? c->fghost_src : c->ghost_src,
+                       c->nscratch ? c->fghost_sign : c->ghost_sign, planes, pstar);
     BX_HIP(hipGetLastError());
     return BX_OK;
 }

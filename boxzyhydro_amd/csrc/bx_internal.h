@@ -67,6 +67,13 @@ struct bx_ctx {
     int32_t *ghost_scratch_src;
     double *ghost_scratch;
     int64_t nscratch;
+    // the same table composed over a pair of set_ghost_cells calls (every call
+    // site of the step makes two in a row): ghost dst[k] <- +-(interior cell
+    // fsrc[k]), exact when each scratch donor's own donor is an interior cell
+    // (fghost_ok); equal to the table above when nscratch == 0
+    int32_t *fghost_src;
+    int32_t *fghost_sign;
+    bool fghost_ok;
 
     // fused-path state
     double *ustar;     // 5 planes: stage-1 result U*
@@ -77,7 +84,7 @@ struct bx_ctx {
     bool fused_unsupported;
     bool ghosts_filled;  // ghost cells hold their donors' values (after any exact step)
     int fused_ly, fused_lz;  // y rows and z planes per wave of the stage kernels
-    int fused_occ;           // x/z kernel built for >= 4 waves/SIMD (register cap) when 4
+    int fused_occ;           // stage kernel variant (bx_tune): 0, 1, 4 two kernels; 10, 11 one tiled kernel
     double *eflux;           // 5 planes: y-term of fluxtmp (FP64)
     bool prof_on;
     hipEvent_t ev[6];   // per stage: before the y kernel, between the two kernels, after the x/z kernel
@@ -86,6 +93,7 @@ struct bx_ctx {
     bool aux_stale;    // u, p planes not materialised since the last fused step
     bool old_stale;    // cons_old not materialised since the last fused step
     int64_t fallbacks;
+    int64_t nfused, nexact;  // steps bx_step took with the fused kernels / the exact sequence
 
     // captured pairs of fused steps (BX_STEP_GRAPH), keyed by the state buffer
     // they start from (the pair returns the state to it)

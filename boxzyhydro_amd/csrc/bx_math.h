@@ -77,6 +77,20 @@ __device__ __forceinline__ T slope_finite(T c0, T c1, T c2)
     return genminmod_finite(tt * (c1 - c0), T(0.5) * (c2 - c0), tt * (c2 - c1));
 }
 
+// slope_finite up to the sign of a zero result: max(mn,0) + min(mx,0) is mn
+// when all three arguments are positive, mx when all are negative, and a zero
+// otherwise.  For use where the slope only enters q + c*s with a canonical
+// (never -0) q, which gives the same value for a zero slope of either sign.
+template <typename T>
+__device__ __forceinline__ T slope_fz(T c0, T c1, T c2)
+{
+    const T tt = T(BX_THETA);
+    const T x1 = tt * (c1 - c0), x2 = T(0.5) * (c2 - c0), x3 = tt * (c2 - c1);
+    const T mn = Arith<T>::fmin(x1, Arith<T>::fmin(x2, x3));
+    const T mx = Arith<T>::fmax(x1, Arith<T>::fmax(x2, x3));
+    return Arith<T>::fmax(mn, T(0)) + Arith<T>::fmin(mx, T(0));
+}
+
 // Primitive variables of one cell as the flux routine sees them.
 template <typename T>
 struct Prim {
@@ -95,6 +109,44 @@ __device__ __forceinline__ void muscl(T q, T lm, T lp, T s, T &minus, T &plus)
 {
     minus = q + T(0.5) * (T(-1) - lm) * s;
     plus = q + T(0.5) * (T(1) - lp) * s;
+}
+
+// Correctly rounded quotients a[i]/b (i < N) that share one reciprocal, for
+// FP64.  The hardware's IEEE division is v_div_scale (numerator and
+// denominator), v_rcp, two Newton steps on the reciprocal, the quotient and
+// its residual, v_div_fmas and v_div_fixup.  When b lies in [2^-300, 2^300]
+// and each a is zero or lies in [2^-299, 2^302] in magnitude, v_div_scale
+// leaves both operands as they are (exponent difference below 768, no
+// denormal reciprocal or quotient, numerator exponent far above 53), VCC is
+// clear and v_div_fixup returns its input for a nonzero quotient, so that
+// sequence is
+//     y0 = rcp(b); y1 = fma(y0, fma(-b, y0, 1), y0); y = fma(y1, fma(-b, y1, 1), y1)
+//     q0 = a*y;    r = fma(-b, q0, a);              q = fma(r, y, q0)
+// and y does not depend on a.  Here the residual is formed negated,
+// r' = fma(b, q0, -a), and q = fma(-r', y, q0): for a nonzero the same bits
+// (round-to-nearest is symmetric), and for a zero the signed zero that
+// v_div_fixup gives (q0 = +-0, r' = +0, q = -0*y + q0 = q0).  A lane with an
+// operand outside the range takes the plain division for all N.
+template <int N>
+__device__ __forceinline__ void div_shared(const double *a, double b, double *q)
+{
+    const double y0 = __builtin_amdgcn_rcp(b);
+    const double y1 = __builtin_fma(y0, __builtin_fma(-b, y0, 1.0), y0);
+    const double y = __builtin_fma(y1, __builtin_fma(-b, y1, 1.0), y1);
+    double mag = 0.0;
+    int emin = 0;
+    for (int i = 0; i < N; i++) {
+        const double q0 = a[i] * y;
+        q[i] = __builtin_fma(-__builtin_fma(b, q0, -a[i]), y, q0);
+        mag += __builtin_fabs(a[i]);
+        const int e = a[i] == 0.0 ? 0 : __builtin_amdgcn_frexp_exp(a[i]);
+        emin = i == 0 ? e : min(emin, e);
+    }
+    // |a[i]| <= sum <= 2^302 (false for NaN or inf), exponent(a[i]) >= -298
+    // for a[i] != 0 (frexp: a = m 2^e, 0.5 <= |m| < 1), b in [2^-300, 2^300]
+    const bool ok = mag <= 0x1p302 && emin >= -298 && b >= 0x1p-300 && b <= 0x1p300;
+    if (__builtin_expect(!ok, 0))
+        for (int i = 0; i < N; i++) q[i] = a[i] / b;
 }
 
 // a/b for double, where a zero numerator is answered without the division:
@@ -146,13 +198,20 @@ __device__ __forceinline__ void face_flux(const Side<T> &L, const Side<T> &R, T 
     const T am = vmax(vmax(zero, -(vr - cr)), -(vl - cl));
     const T apam = ap * am;
     const T den = ap + am;
+    if constexpr (sizeof(T) == 8 && !ZSKIP) {
+        double num[5];
 #pragma unroll
-    for (int i = 0; i < 5; i++) {
-        const T num = ap * fl[i] + am * fr[i] - apam * (sr[i] - sl[i]);
-        if constexpr (ZSKIP && sizeof(T) == 8)
-            F[i] = qdiv(num, den);
-        else
-            F[i] = Arith<T>::div(num, den);
+        for (int i = 0; i < 5; i++) num[i] = ap * fl[i] + am * fr[i] - apam * (sr[i] - sl[i]);
+        div_shared<5>(num, den, F);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            const T num = ap * fl[i] + am * fr[i] - apam * (sr[i] - sl[i]);
+            if constexpr (ZSKIP && sizeof(T) == 8)
+                F[i] = qdiv(num, den);
+            else
+                F[i] = Arith<T>::div(num, den);
+        }
     }
 }
 

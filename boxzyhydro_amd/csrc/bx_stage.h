@@ -97,18 +97,22 @@ __device__ __forceinline__ T comp(const W6<T> &w, int j)
 
 // interface states of one cell in direction d (flux.f90:253-351, 391-479):
 // slopes from (wm, wc, wp) unless the cell is a ghost (zero slopes).
+// The reference forms each state as (q + c*s) + 0 (the transverse terms are
+// exact zeros); that equals (q + 0) + c*s bit for bit, so the centre values
+// are made canonical once (no -0) and a zero slope of either sign gives the
+// same state (slope_fz).
 template <int d, typename T>
 __device__ __forceinline__ void dir_states(const W6<T> &wm, const W6<T> &wc, const W6<T> &wp, T sound, bool ghost,
                                            const StageArgs &a, Side<T> &mi, Side<T> &pl)
 {
     T su0 = T(0), su1 = T(0), su2 = T(0), sp = T(0), sd = T(0), se = T(0);
     if (!ghost) {
-        su0 = slope_finite(wm.u0, wc.u0, wp.u0);
-        su1 = slope_finite(wm.u1, wc.u1, wp.u1);
-        su2 = slope_finite(wm.u2, wc.u2, wp.u2);
-        sp = slope_finite(wm.p, wc.p, wp.p);
-        sd = slope_finite(wm.d, wc.d, wp.d);
-        se = slope_finite(wm.e, wc.e, wp.e);
+        su0 = slope_fz(wm.u0, wc.u0, wp.u0);
+        su1 = slope_fz(wm.u1, wc.u1, wp.u1);
+        su2 = slope_fz(wm.u2, wc.u2, wp.u2);
+        sp = slope_fz(wm.p, wc.p, wp.p);
+        sd = slope_fz(wm.d, wc.d, wp.d);
+        se = slope_fz(wm.e, wc.e, wp.e);
     }
     const T ud = comp(wc, d);
     const T dt = T(a.dt);
@@ -116,18 +120,19 @@ __device__ __forceinline__ void dir_states(const W6<T> &wm, const W6<T> &wc, con
     const T lm = div_by(vmin(ud - sound, T(0)) * dt, a.ds[d], a.inv_ds[d], a.ds_pow2[d]);
     const T cm = T(0.5) * (T(-1) - lm), cp = T(0.5) * (T(1) - lp);
     const T z = T(0);  // transverse terms: 0*cc with cc=dt*half/ds > 0
-    mi.d = (wc.d + cm * sd) + z;
-    pl.d = (wc.d + cp * sd) + z;
-    mi.v[0] = (wc.u0 + cm * su0) + z;
-    pl.v[0] = (wc.u0 + cp * su0) + z;
-    mi.v[1] = (wc.u1 + cm * su1) + z;
-    pl.v[1] = (wc.u1 + cp * su1) + z;
-    mi.v[2] = (wc.u2 + cm * su2) + z;
-    pl.v[2] = (wc.u2 + cp * su2) + z;
-    mi.p = (wc.p + cm * sp) + z;
-    pl.p = (wc.p + cp * sp) + z;
-    mi.e = (wc.e + cm * se) + z;
-    pl.e = (wc.e + cp * se) + z;
+    const T d0 = wc.d + z, v0 = wc.u0 + z, v1 = wc.u1 + z, v2 = wc.u2 + z, p0 = wc.p + z, e0 = wc.e + z;
+    mi.d = d0 + cm * sd;
+    pl.d = d0 + cp * sd;
+    mi.v[0] = v0 + cm * su0;
+    pl.v[0] = v0 + cp * su0;
+    mi.v[1] = v1 + cm * su1;
+    pl.v[1] = v1 + cp * su1;
+    mi.v[2] = v2 + cm * su2;
+    pl.v[2] = v2 + cp * su2;
+    mi.p = p0 + cm * sp;
+    pl.p = p0 + cp * sp;
+    mi.e = e0 + cm * se;
+    pl.e = e0 + cp * se;
     mi.g = T(a.gam);  // slope(gamma,gamma,gamma)=0: gamma + (+-0) + 0 = gamma
     pl.g = T(a.gam);
 }
@@ -171,13 +176,22 @@ __device__ __forceinline__ W6<S> prims(const StageArgs &a, const Raw<S> &v, unsi
     if constexpr (sizeof(S) == 8) {
         const double r = v.r, m0 = v.m0, m1 = v.m1, m2 = v.m2, E = v.E;
         w.d = r;
-        w.u0 = m0 / r;
-        w.u1 = m1 / r;
-        w.u2 = m2 / r;
+        // u = m/rho (velocity.f90:21) and, in stage 1, the kinetic energy of
+        // the source re-round: quotients by rho with one shared reciprocal
+        double num[4] = {m0, m1, m2, 0.0}, quo[4];
+        if (STAGE == 1) {
+            num[3] = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2);
+            div_shared<4>(num, r, quo);
+        } else {
+            div_shared<3>(num, r, quo);
+        }
+        w.u0 = quo[0];
+        w.u1 = quo[1];
+        w.u2 = quo[2];
         if (STAGE == 1) {
             // source(0) energy re-round with g = -0 (source.f90:176-182), then
             // state at main.f90:314 with u = m/rho, velocity at :318
-            const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r;
+            const double ek = quo[3];
             const double E1 = vmax(E - ek, a.seps) + ek;
             w.e = vmax(E1 - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
             w.p = w.e * a.gm1;

@@ -83,6 +83,95 @@ __device__ __forceinline__ void ydir_wave(const StageArgs &a, S *__restrict__ Ey
     if (flg) atomicOr(a.flags, flg);
 }
 
+// The stage update of one output cell c (uniform plane offset c + per-lane
+// byte offset coff) from U (ui), U^n (uo, stage 2) and fluxtmp (ft), and
+// everything pointwise up to the next stage: stage 1 writes U* and the
+// stale-velocity pressure p* (q0: the cell's primitives of the stage input);
+// stage 2 writes U^{n+1} and accumulates the next step's Courant rate.
+template <int STAGE, typename S>
+__device__ __forceinline__ void stage_update(const StageArgs &a, const double *ui, const double *uo, const double *ft,
+                                             const W6<S> &q0, int64_t c, unsigned coff, unsigned int &flg,
+                                             double &rmax)
+{
+    const int64_t n = a.n;
+    double U[5];
+    if (STAGE == 1) {
+        // cons = cons + fluxtmp (flux.f90:205) with cons = source(0)(U^n)
+        const double r0 = ui[0], m0 = ui[1], m1 = ui[2], m2 = ui[3];
+        const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r0;
+        const double E1 = vmax(ui[4] - ek, a.seps) + ek;
+        U[0] = r0 + ft[0];
+        U[1] = m0 + ft[1];
+        U[2] = m1 + ft[2];
+        U[3] = m2 + ft[3];
+        U[4] = E1 + ft[4];
+    } else {
+        // cons = half*(cons_old+cons+fluxtmp) (flux.f90:213), cons_old = source(0)(U^n)
+        const double r0 = uo[0], m0 = uo[1], m1 = uo[2], m2 = uo[3];
+        const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r0;
+        const double E1 = vmax(uo[4] - ek, a.seps) + ek;
+        const double old[5] = {r0, m0, m1, m2, E1};
+        for (int i = 0; i < 5; i++) U[i] = 0.5 * (old[i] + ui[i] + ft[i]);
+    }
+    // floors, flux.f90:225-228
+    if (!(U[0] > a.srho) || U[4] < 0.0) flg |= FLAG_FLOOR;
+    U[0] = vmax(U[0], a.srho);
+    if (U[4] < 0.0) U[4] = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0] + a.seps;
+    if (STAGE == 1) {
+        // p* = state() at main.f90:353 on U* with the velocity of :318
+        if (U[0] < 1e-4) flg |= FLAG_LOWRHO;
+        S Us[5];
+        for (int i = 0; i < 5; i++) Us[i] = (S)U[i];
+        const double r1 = Us[0], E1 = Us[4];
+        const double w0 = q0.u0, w1 = q0.u1, w2 = q0.u2;
+        const double ekin = r1 * 0.5 * (w0 * w0 + w1 * w1 + w2 * w2);
+        if (!(E1 - ekin > a.seps)) flg |= FLAG_FLOOR;
+        elem<S>(a.pout, c, coff) = (S)(vmax(E1 - ekin, a.seps) * a.gm1);
+        for (int i = 0; i < 5; i++) elem<S>(a.uout, (int64_t)i * n + c, coff) = Us[i];
+    } else {
+        // main.f90:374-515: state, velocity, source(1), state, velocity
+        if (U[0] < 1e-4) flg |= FLAG_LOWRHO;
+        // u = m/rho and the kinetic energy of the source re-round, quotients
+        // by rho with one shared reciprocal
+        const double num[4] = {U[1], U[2], U[3], 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3])};
+        double quo[4];
+        div_shared<4>(num, U[0], quo);
+        const double v0 = quo[0], v1 = quo[1], v2 = quo[2];
+        if (fabs(v0) > a.vlimit || fabs(v1) > a.vlimit || fabs(v2) > a.vlimit) flg |= FLAG_CLAMP;
+        const double ek = quo[3];
+        if (!(U[4] - ek > a.seps)) flg |= FLAG_FLOOR;
+        U[4] = vmax(U[4] - ek, a.seps) + ek;
+        S Us[5];
+        for (int i = 0; i < 5; i++) {
+            Us[i] = (S)U[i];
+            elem<S>(a.uout, (int64_t)i * n + c, coff) = Us[i];
+        }
+        // next step: state (main.f90:232) and courant rate (courant.f90:20-21)
+        double p, w0, w1, w2, rr;
+        if constexpr (sizeof(S) == 8) {
+            rr = U[0];
+            w0 = v0;
+            w1 = v1;
+            w2 = v2;
+            const double ekin = U[0] * 0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
+            p = vmax(U[4] - ekin, a.seps) * a.gm1;
+        } else {
+            rr = Us[0];
+            const double inv = 1.0 / rr;
+            w0 = (double)Us[1] * inv;
+            w1 = (double)Us[2] * inv;
+            w2 = (double)Us[3] * inv;
+            const double ekin = 0.5 * ((double)Us[1] * w0 + (double)Us[2] * w1 + (double)Us[3] * w2);
+            p = vmax((double)Us[4] - ekin, a.seps) * a.gm1;
+        }
+        const double snd1 = sqrt(a.gam * p / rr);
+        const double rt = div_by(snd1 + fabs(w0), a.ds[0], a.inv_ds[0], a.ds_pow2[0]) +
+                          div_by(snd1 + fabs(w1), a.ds[1], a.inv_ds[1], a.ds_pow2[1]) +
+                          div_by(snd1 + fabs(w2), a.ds[2], a.inv_ds[2], a.ds_pow2[2]);
+        rmax = vmax(rmax, rt);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // x/z kernel: each wave holds 64 consecutive cells of the row-major x-y plane
 // (lanes 2..61 are outputs where interior, 0,1,62,63 the x-halo; a wave may
@@ -211,76 +300,7 @@ __device__ __forceinline__ void xz_wave(const StageArgs &a, const S *__restrict_
                 const double s = (a.ayz * ddx[i] + ey[i]) + a.axy * dz;
                 ft[i] = -div_by(s * a.dt, a.vol, a.inv_vol, a.vol_pow2);
             }
-            double U[5];
-            if (STAGE == 1) {
-                // cons = cons + fluxtmp (flux.f90:205) with cons = source(0)(U^n)
-                const double r0 = ui[0], m0 = ui[1], m1 = ui[2], m2 = ui[3];
-                const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r0;
-                const double E1 = vmax(ui[4] - ek, a.seps) + ek;
-                U[0] = r0 + ft[0];
-                U[1] = m0 + ft[1];
-                U[2] = m1 + ft[2];
-                U[3] = m2 + ft[3];
-                U[4] = E1 + ft[4];
-            } else {
-                // cons = half*(cons_old+cons+fluxtmp) (flux.f90:213), cons_old = source(0)(U^n)
-                const double r0 = uo[0], m0 = uo[1], m1 = uo[2], m2 = uo[3];
-                const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r0;
-                const double E1 = vmax(uo[4] - ek, a.seps) + ek;
-                const double old[5] = {r0, m0, m1, m2, E1};
-                for (int i = 0; i < 5; i++) U[i] = 0.5 * (old[i] + ui[i] + ft[i]);
-            }
-            // floors, flux.f90:225-228
-            if (!(U[0] > a.srho) || U[4] < 0.0) flg |= FLAG_FLOOR;
-            U[0] = vmax(U[0], a.srho);
-            if (U[4] < 0.0) U[4] = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0] + a.seps;
-            if (STAGE == 1) {
-                // p* = state() at main.f90:353 on U* with the velocity of :318
-                if (U[0] < 1e-4) flg |= FLAG_LOWRHO;
-                S Us[5];
-                for (int i = 0; i < 5; i++) Us[i] = (S)U[i];
-                const double r1 = Us[0], E1 = Us[4];
-                const double w0 = q0.u0, w1 = q0.u1, w2 = q0.u2;
-                const double ekin = r1 * 0.5 * (w0 * w0 + w1 * w1 + w2 * w2);
-                if (!(E1 - ekin > a.seps)) flg |= FLAG_FLOOR;
-                elem<S>(a.pout, c, coff) = (S)(vmax(E1 - ekin, a.seps) * a.gm1);
-                for (int i = 0; i < 5; i++) elem<S>(a.uout, (int64_t)i * n + c, coff) = Us[i];
-            } else {
-                // main.f90:374-515: state, velocity, source(1), state, velocity
-                if (U[0] < 1e-4) flg |= FLAG_LOWRHO;
-                const double v0 = U[1] / U[0], v1 = U[2] / U[0], v2 = U[3] / U[0];
-                if (fabs(v0) > a.vlimit || fabs(v1) > a.vlimit || fabs(v2) > a.vlimit) flg |= FLAG_CLAMP;
-                const double ek = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0];
-                if (!(U[4] - ek > a.seps)) flg |= FLAG_FLOOR;
-                U[4] = vmax(U[4] - ek, a.seps) + ek;
-                S Us[5];
-                for (int i = 0; i < 5; i++) {
-                    Us[i] = (S)U[i];
-                    elem<S>(a.uout, (int64_t)i * n + c, coff) = Us[i];
-                }
-                // next step: state (main.f90:232) and courant rate (courant.f90:20-21)
-                double p, w0, w1, w2, rr;
-                if constexpr (sizeof(S) == 8) {
-                    rr = U[0];
-                    w0 = v0;
-                    w1 = v1;
-                    w2 = v2;
-                    const double ekin = U[0] * 0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
-                    p = vmax(U[4] - ekin, a.seps) * a.gm1;
-                } else {
-                    rr = Us[0];
-                    const double inv = 1.0 / rr;
-                    w0 = (double)Us[1] * inv;
-                    w1 = (double)Us[2] * inv;
-                    w2 = (double)Us[3] * inv;
-                    const double ekin = 0.5 * ((double)Us[1] * w0 + (double)Us[2] * w1 + (double)Us[3] * w2);
-                    p = vmax((double)Us[4] - ekin, a.seps) * a.gm1;
-                }
-                const double snd1 = sqrt(a.gam * p / rr);
-                const double rt =
-                    (snd1 + fabs(w0)) / a.ds[0] + (snd1 + fabs(w1)) / a.ds[1] + (snd1 + fabs(w2)) / a.ds[2];
-                rmax = vmax(rmax, rt);
-            }
+            stage_update<STAGE, S>(a, ui, uo, ft, q0, c, coff, flg, rmax);
         }
         if (!a.flatz) {
             for (int i = 0; i < 5; i++) Fz_prev[i] = Fz1[i];

@@ -241,8 +241,10 @@ class HydroGrid:
         return dts[:taken.value].copy() if record_dt else np.zeros(0)
 
     def tune(self, y_chunk: int, z_chunk: int, occupancy: int = 0) -> None:
-        """Rows per wave of the y-kernel, planes per wave of the x/z kernel and
-        its occupancy variant (0 or 4) for the fused step (results unchanged)."""
+        """Work split of the fused step's stage kernels (bx_tune): rows per
+        wave of the y-kernel, planes per wave of the x/z kernel or of the
+        tiled stage kernel, and the variant (0, 1, 4: two kernels per stage;
+        10, 11: one tiled kernel).  Results unchanged."""
         nv.check(nv.lib().bx_tune(self._h, y_chunk, z_chunk, occupancy), "bx_tune")
 
     def profile(self, enable: int = -1):
@@ -273,6 +275,13 @@ class HydroGrid:
         c = ctypes.c_int64()
         nv.check(nv.lib().bx_graph_step_count(self._h, ctypes.byref(c)), "bx_graph_step_count")
         return c.value
+
+    def step_counts(self):
+        """(steps taken with the fused kernels, steps taken with the exact
+        sequence) since the grid was created (bx_step_counts)."""
+        f, e = ctypes.c_int64(), ctypes.c_int64()
+        nv.check(nv.lib().bx_step_counts(self._h, ctypes.byref(f), ctypes.byref(e)), "bx_step_counts")
+        return f.value, e.value
 
     def fallback_count(self) -> int:
         c = ctypes.c_int64()

@@ -98,6 +98,7 @@ _SIGS = {
     "bx_step_static": [_VP, _VP, _VP, ctypes.c_double, _VP, _VP, _VP],
     "bx_fallback_count": [_VP, ctypes.POINTER(ctypes.c_int64)],
     "bx_graph_step_count": [_VP, ctypes.POINTER(ctypes.c_int64)],
+    "bx_step_counts": [_VP, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
     "bx_tune": [_VP, _I32, _I32, _I32],
     "bx_profile": [_VP, _I32, _DP, ctypes.POINTER(ctypes.c_int64)],
     "bx_sync": [_VP],

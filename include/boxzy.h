@@ -147,16 +147,28 @@ int bx_fallback_count(const bx_ctx *ctx, int64_t *count);
  * that were undone are not counted) since the context was created. */
 int bx_graph_step_count(const bx_ctx *ctx, int64_t *count);
 
-/* Work split of the fused stage kernels: y rows per wave of the y-direction
- * kernel, z planes per wave of the x/z kernel, and the x/z kernel variant
- * (0: default; 1: y-term loaded one plane early; 4: built for four waves per
- * SIMD).  Used by the autotuner; results do not change. */
+/* Steps bx_step has taken since the context was created: with the fused
+ * kernels (including graph replays; steps redone are not counted) and with
+ * the exact call sequence (the first step from a set-up, steps redone, and
+ * every step of a grid the fused kernels do not cover).  Either pointer may
+ * be NULL.  A diagnostic with no reference counterpart. */
+int bx_step_counts(const bx_ctx *ctx, int64_t *fused, int64_t *exact);
+
+/* Work split of the fused stage kernels.  variant 11 (the FP64 default) and
+ * 10: one kernel per stage over x-y tiles of 64 x 8 (11) or 64 x 4 (10)
+ * cells marching z_chunk planes in z (y_chunk unused).  Variants 0, 1, 4
+ * (1 is the mixed form's default): two kernels per stage, a y-direction
+ * kernel of y_chunk rows per wave and an x/z kernel of z_chunk planes per
+ * wave (1: y-term loaded one plane early; 4: built for four waves per SIMD).
+ * Used by the autotuner; results do not change. */
 int bx_tune(bx_ctx *ctx, int32_t y_chunk, int32_t z_chunk, int32_t variant);
 
 /* Event timing of the fused stage kernels on the context's stream.
  * enable=1 starts (and clears), 0 stops, -1 only reads.  kernel_ms[4]
  * receives the summed durations of the y-direction and the x/z kernel of
- * stage 1, then of stage 2, over *steps steps. */
+ * stage 1, then of stage 2, over *steps steps; with one kernel per stage
+ * (variants 10, 11) the y-direction entries are ~0 and the x/z entries hold
+ * the stage kernels. */
 int bx_profile(bx_ctx *ctx, int32_t enable, double *kernel_ms, int64_t *steps);
 
 /* ------------------------------------------------------------------------

@@ -280,3 +280,79 @@ def test_adjoint_refused_forward_leaves_nothing_to_sweep():
     low[int(interior[len(interior) // 2])] = 5e-5
     refused(lambda: st.forward(low, p0), nv.BX_ERR_UNSUPPORTED)
     refused(lambda: st.backward(g), nv.BX_ERR_STATE)
+
+
+@pytest.mark.gpu
+def test_adjoint_two_rollouts_backward_of_the_first():
+    """Two applications on one stepper, then the backward of the first
+    output: the context holds the second forward's checkpoints, so the first
+    node runs its own forward again (from its saved inputs and dt sequence)
+    and returns the same bits as a single application."""
+    import torch
+
+    from boxzyhydro_amd.adjoint import hydro_function
+
+    prm, st, rho0, p0, tgt, interior = _setup(n=16, nsteps=8, every=3, seed=2)
+    f = hydro_function()
+
+    def loss_grad(apply_second):
+        rho = rho0.clone().requires_grad_(True)
+        p = p0.clone().requires_grad_(True)
+        r1 = f.apply(rho, p, st)
+        if apply_second:
+            r2 = f.apply(rho0 * 1.01, p0 * 0.99, st)  # another rollout, not part of the loss
+            assert not torch.equal(r1, r2)
+        loss = (0.5 * (r1 - tgt) ** 2 * interior).sum()
+        loss.backward()
+        return rho.grad.clone(), p.grad.clone()
+
+    g_single = loss_grad(False)
+    g_two = loss_grad(True)
+    assert torch.equal(g_single[0], g_two[0]) and torch.equal(g_single[1], g_two[1])
+
+
+@pytest.mark.gpu
+def test_adjoint_refusal_mid_run_restores_the_start():
+    """A forward refused after its steps ran (a diverging flow whose density
+    falls below 1e-4 within the run) puts the starting state back (its
+    ghosts filled by the forward's first call) and marks u, p and the Courant
+    rate stale: a download matches that state, and fused steps from it match
+    a fresh context started from the same state."""
+    import ctypes
+
+    import torch
+
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+    from boxzyhydro_amd.params import HydroParams
+
+    prm = HydroParams(nx=44, ny=12, nz=12, dx=1.0, dy=1.0, dz=1.0, gammafix=1.4, cfl=0.5, endtime=1e30,
+                      no_outflow_xl=False, no_outflow_xr=False, no_outflow_yl=False, no_outflow_yr=False,
+                      no_outflow_zl=False, no_outflow_zr=False)
+    x, _, _ = ics.cell_centres(prm)
+    shp = prm.shape
+    rho0 = 0.006
+    vx = np.broadcast_to(np.sign(x)[None, None, :] * 1.0, shp)
+    ic = ics._assemble(prm, np.full(shp, rho0), vx, np.zeros(shp), np.zeros(shp), np.full(shp, 0.01 * rho0))
+    g = HydroGrid(prm, ic)
+    rc = nv.lib().bx_adjoint_forward(g._h, 40, 10, None, None)
+    assert rc == nv.BX_ERR_UNSUPPORTED and b"rho<1e-4" in nv.lib().bx_last_error()
+    want = HydroGrid(prm, ic)
+    want.set_ghost_cells()  # the forward's own first call
+    w = want.download()["cons"]
+    out = g.download()
+    assert np.array_equal(out["cons"].view(np.uint64), w.view(np.uint64))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        assert np.array_equal(out["u"].view(np.uint64), (out["cons"][:, 1:4] / out["cons"][:, :1]).view(np.uint64))
+    s0 = torch.empty(5 * g.ngrid, dtype=torch.float64, device="cuda")
+    nv.check(nv.lib().bx_cons_device(g._h, ctypes.c_void_p(s0.data_ptr()), 0), "bx_cons_device")
+    fresh = HydroGrid(prm, None)
+    nv.check(nv.lib().bx_cons_device(fresh._h, ctypes.c_void_p(s0.data_ptr()), 1), "bx_cons_device")
+    d1 = g.run(20, endtime=1e30, mode=nv.BX_STEP_FUSED)
+    d2 = fresh.run(20, endtime=1e30, mode=nv.BX_STEP_FUSED)
+    assert np.array_equal(d1.view(np.uint64), d2.view(np.uint64))
+    o1, o2 = g.download(), fresh.download()
+    for k in ("cons", "u", "p"):
+        assert np.array_equal(o1[k].view(np.uint64), o2[k].view(np.uint64)), k
+    for h in (g, want, fresh):
+        h.close()

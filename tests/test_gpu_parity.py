@@ -112,8 +112,12 @@ def test_box64_matches_oracle():
 @pytest.mark.parametrize("name", ["box24_outflow", "sod_t02", "stress8", "box12_walls", "flatz_walls",
                                   "riemann4_1", "box12_outflow", "readme_sod_y"])
 def test_fused_step_matches_reference(golden, name):
-    """The fused two-kernel step (with its exact-sequence fallback) gives the
-    reference's dt sequence and final state bit for bit."""
+    """The fused step (with its exact-sequence fallback) gives the
+    reference's dt sequence and final state bit for bit.  The path taken is
+    checked too: box12_walls (walls on every side of a 3-D box, ghost donors
+    chained beyond a pair of set_ghost_cells calls) takes the exact sequence
+    every step; every other case takes the fused kernels after the first
+    step, except for the steps it redoes (stress8 forces irregular cells)."""
     from boxzyhydro_amd import native as nv
 
     rec = golden[name]
@@ -125,10 +129,20 @@ def test_fused_step_matches_reference(golden, name):
     out = g.download()
     for k in ("cons", "u", "p"):
         assert sha(out[k]) == rec["final_sha256"][k], k
-    print(f"  fallbacks {g.fallback_count()}", flush=True)
+    fused, exact = g.step_counts()
+    fb = g.fallback_count()
+    print(f"  fused steps {fused}, exact steps {exact}, fallbacks {fb}", flush=True)
+    assert fused + exact == rec["steps"]
+    if name == "box12_walls":
+        assert fused == 0
+    elif name == "stress8":
+        assert exact >= 1 + fb  # a step from an irregular state is taken exactly without a fused attempt
+    else:
+        assert exact == 1 and fb == 0
 
 
-@pytest.mark.parametrize("tile", [(4, 7, 0), (32, 32, 4), (13, 5, 1), (64, 64, 4)])
+@pytest.mark.parametrize("tile", [(4, 7, 0), (32, 32, 4), (13, 5, 1), (64, 64, 4), (64, 7, 11), (64, 64, 10),
+                                  (64, 3, 11)])
 def test_fused_tiles_match_exact_box64(tile):
     """Every tile shape of the stage kernels gives the exact sequence's result
     on a 64^3 box (5 steps, first one exact from the set-up)."""
