@@ -40,7 +40,16 @@ __global__ __launch_bounds__(64 * (TY + 4)) void k_stage(StageArgs a0)
 {
     __shared__ TileLds<S, TY> lds;
     const StageArgs a = with_dt(a0);
-    tile_stage<STAGE, S, TY>(a, lds, blockIdx.x, blockIdx.y, blockIdx.z);
+    // Workgroups go to the 8 XCDs in turn (workgroup id mod 8).  Give each
+    // XCD a contiguous range of tiles, x fastest, so that the tiles in flight
+    // on one XCD are neighbours and share their halo rows and boundary cache
+    // lines in that XCD's L2.
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned T = gx * gy * gridDim.z;
+    const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned xcd = L & 7u, slot = L >> 3, q = T >> 3, rem = T & 7u;
+    const unsigned t = xcd * q + (xcd < rem ? xcd : rem) + slot;
+    tile_stage<STAGE, S, TY>(a, lds, t % gx, (t / gx) % gy, t / (gx * gy));
 }
 
 
