@@ -40,24 +40,66 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(prm, ic, steps: int, threads: int):
+def host_cores():
+    """Physical cores in this process's affinity mask (one per core id and
+    package), and the CPU model from lscpu."""
+    import subprocess
+
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology"
+        try:
+            with open(f"{base}/core_id") as f:
+                core = f.read().strip()
+            with open(f"{base}/physical_package_id") as f:
+                pkg = f.read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", str(c)))
+    info = {"affinity_cpus": len(cpus), "physical_cores": len(cores)}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core",
+                             "NUMA node(s)"):
+                info[k.strip()] = v.strip()
+    except Exception as e:  # lscpu is informational only
+        info["lscpu_error"] = str(e)
+    return info
+
+
+def cpu_baseline(prm, ic, steps: int, threads: int = 0):
     """Time the reference step on the host: the reference's own subroutines if
-    built (oracle/_ref), else the C restatement (oracle/build)."""
+    built (oracle/_ref), else the C restatement (oracle/build).  Threads: the
+    physical cores of the affinity mask (unless given), bound with
+    OMP_PROC_BIND=close and OMP_PLACES=cores; two runs of `steps` steps, the
+    hot path alone and with the per-step conservation diagnostics that the
+    reference's main loop runs every step (main.f90:222-223)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
+    host = host_cores()
+    threads = threads or host["physical_cores"]
+    bind = {"OMP_PROC_BIND": "close", "OMP_PLACES": "cores"}
     cells = prm.interior_cells * steps
     if po.have_reference():
         try:
             wd = "/dev/shm" if os.path.isdir("/dev/shm") else None
-            r = po.run_reference(prm, ic, steps, threads=threads, workdir=wd)
+            r = po.run_reference(prm, ic, steps, threads=threads, workdir=wd, env_extra=bind)
+            rd = po.run_reference(prm, ic, steps, threads=threads, workdir=wd, env_extra=bind, diag=1)
             return {"value": cells / r["wall_s"], "unit": "cell-updates/s", "cores": r["threads"],
                     "kind": "reference",
+                    "with_diagnostics": cells / rd["wall_s"],
                     "sample": f"{steps} steps of the same {prm.nx - 4}^3 box, the reference's Fortran "
-                              f"subroutines (amdflang -O3 -fopenmp) driven through main.f90's step, "
-                              f"wall time of the step loop"}
+                              f"subroutines (amdflang -O3 -fopenmp) driven through main.f90's step, wall "
+                              f"time of the step loop (omp_get_wtime); with_diagnostics adds "
+                              f"conservation_diagnostic and get_etot every step",
+                    "omp": dict(bind, OMP_NUM_THREADS=str(threads)), "host": host}
         except Exception as e:  # fall back to the restatement
             log(f"reference baseline unavailable: {e}")
+    os.environ.update(bind)
     po.set_threads(threads)
     og = po.OracleGrid(prm, ic)
     t0 = time.perf_counter()
@@ -65,7 +107,8 @@ def cpu_baseline(prm, ic, steps: int, threads: int):
     dt = time.perf_counter() - t0
     return {"value": cells / dt, "unit": "cell-updates/s", "cores": po.get_threads(), "kind": "port",
             "sample": f"{steps} steps of the same {prm.nx - 4}^3 box, C/OpenMP restatement of the reference "
-                      f"step (oracle/boxzy_oracle.c)"}
+                      f"step (oracle/boxzy_oracle.c)", "omp": dict(bind, OMP_NUM_THREADS=str(threads)),
+            "host": host}
 
 
 def timed_run(prm, ic, precision, steps, warmup, tile=None, mode=None):
@@ -137,36 +180,44 @@ def graph_cost(steps_small: int, steps_box: int, n: int):
 
 def mixed_study(prm, ic, steps):
     """The mixed form against the FP64 step on the same box: cell-updates/s of
-    each and the per-field L1/Linf error of the mixed state after `steps`."""
+    each (graph replays after the first, exact step) and the per-field L1 and
+    Linf error of the mixed state against the FP64 GPU run after `steps`
+    steps, relative to the FP64 field's L1 norm and max magnitude."""
     import numpy as np
 
     from boxzyhydro_amd import ics
     from boxzyhydro_amd import native as nv
     from boxzyhydro_amd.hydro import HydroGrid
 
-    res = {"workload": f"box {prm.nx - 4}^3, {steps} steps", "fields": {}}
+    res = {"workload": f"box {prm.nx - 4}^3 (same generator), {steps} steps, FP64 and mixed from the same "
+                       f"set-up", "fields": {}}
     states = {}
     for name, prec in (("fp64", nv.BX_PREC_FP64), ("mixed", nv.BX_PREC_MIXED)):
         g = HydroGrid(prm, ic, device=0, precision=prec)
         g.run(1, mode=nv.BX_STEP_FUSED)  # first step from the set-up (exact)
         g.sync()
         t0 = time.perf_counter()
-        g.run(steps - 1, mode=nv.BX_STEP_FUSED)
+        g.run(steps - 1, mode=nv.BX_STEP_GRAPH)
         g.sync()
         dt = time.perf_counter() - t0
         res[name] = {"cell_updates_per_s": prm.interior_cells * (steps - 1) / dt, "time": g.time,
                      "fallbacks": g.fallback_count()}
         states[name] = g.download()
+        g.close()
         del g
     interior = ics.boundary_flags(prm).reshape(-1) == 0
     a, b = states["fp64"], states["mixed"]
-    fields = {"rho": (a["cons"][:, 0], b["cons"][:, 0]), "p": (a["p"], b["p"]),
-              "mx": (a["cons"][:, 1], b["cons"][:, 1]), "E": (a["cons"][:, 4], b["cons"][:, 4])}
-    for k, (x, y) in fields.items():
-        x, y = x[interior], y[interior]
-        res["fields"][k] = {"l1_rel": float(np.abs(x - y).sum() / np.abs(x).sum()),
-                            "linf_rel": float(np.abs(x - y).max() / np.abs(x).max())}
+    names = ("rho", "mx", "my", "mz", "E")
+    for j, k in enumerate(names):
+        x, y = a["cons"][interior, j], b["cons"][interior, j]
+        d = np.abs(x - y)
+        res["fields"][k] = {"l1_rel": float(d.sum() / np.abs(x).sum()), "linf_rel": float(d.max() / np.abs(x).max())}
+    x, y = a["p"][interior], b["p"][interior]
+    d = np.abs(x - y)
+    res["fields"]["p"] = {"l1_rel": float(d.sum() / np.abs(x).sum()), "linf_rel": float(d.max() / np.abs(x).max())}
     res["speedup"] = res["mixed"]["cell_updates_per_s"] / res["fp64"]["cell_updates_per_s"]
+    res["mixed"]["step_hbm_roofline_frac"] = res["mixed"]["cell_updates_per_s"] * B_STEP_MIXED / HBM_PEAK
+    res["fp64"]["step_hbm_roofline_frac"] = res["fp64"]["cell_updates_per_s"] * B_STEP_FP64 / HBM_PEAK
     return res
 
 
@@ -196,7 +247,9 @@ def batched(n_tubes: int, reps: int):
                         f"t_end 0.1, FP64, one persistent launch replayed from a hipGraph",
             "ms": ms, "steps_min": int(steps.min()), "steps_max": int(steps.max()),
             "row_cell_updates_per_s": row_updates / (ms * 1e-3),
-            "grid_cell_updates_per_s": full_updates / (ms * 1e-3),
+            "grid_equivalent_cell_updates_per_s": full_updates / (ms * 1e-3),
+            "grid_equivalent_note": "the L x 4 x 4 grids' cell-updates; their 16 rows are identical and "
+                                    "computed once, so this is not a throughput (row_cell_updates_per_s is)",
             "tile": {"waves_per_block": best[0], "blocks_per_cu": best[1]},
             "tile_ms": {f"{c[0]}x{c[1]}": t * 1e3 for c, t in res},
             "longest_tube_steps": int(steps.max()),
@@ -455,8 +508,10 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--mode", choices=["fused", "exact"], default="fused")
     ap.add_argument("--tile", type=int, nargs=3, default=None, metavar=("YCHUNK", "ZCHUNK", "VARIANT"))
-    ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: the physical cores of the affinity mask")
+    ap.add_argument("--mixed-size", type=int, default=384, help="box of the mixed-form line (config: 384)")
+    ap.add_argument("--mixed-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision line")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched shock tubes")
@@ -562,29 +617,41 @@ def main() -> int:
                    "fallback_steps": fb},
         "step_hbm_roofline_frac": value * B_STEP_FP64 / HBM_PEAK,
     }
-    xz2 = per[3] * 1e-3
+    variant = args.tile[2] if args.tile else 11
+    kname = ("bx::k_stage<2, double, %d> (corrector stage: x/y/z faces, update, next Courant rate)"
+             % (8 if variant == 11 else 4)) if variant >= 10 else \
+        "bx::k_xz<2, double, ...> (corrector stage, x/z faces + update)"
+    k2 = per[3] * 1e-3
+    kernel_alg = B_XZ2_FP64 * cells  # U*, U^n read, U^{n+1} written: 5 FP64 fields each
     traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_k_xz2_box{args.size}.json")
-    if os.path.exists(tf):
-        with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    achieved = B_XZ2_FP64 * cells / xz2
-    out["roofline"] = {"kernel": "bx::k_xz<2, double, ...> (corrector stage, x/z faces + update)", "bound": "hbm",
-                       "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK, "traffic": traffic,
-                       "alg_bytes_per_launch": B_XZ2_FP64 * cells, "avg_launch_ms": per[3],
-                       "kernel_avg_ms": {"k_ydir<1>": per[0], "k_xz<1>": per[1], "k_ydir<2>": per[2],
-                                         "k_xz<2>": per[3]},
-                       "launches_timed": nprof}
+    tfile = os.path.join("profiles", f"traffic_fused_step_box{args.size}.json")
+    if os.path.exists(os.path.join(ROOT, tfile)):
+        with open(os.path.join(ROOT, tfile)) as f:
+            tj = json.load(f)
+        traffic = {"step_bytes": tj.get("step_hbm_bytes"), "kernel_bytes": tj.get("corrector_hbm_bytes"),
+                   "source": f"{tfile}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes of this workload, "
+                             f"scaled by a plain copy's counters (measured there, not in this run)"}
+    step_achieved = value * B_STEP_FP64
+    out["roofline"] = {"scope": "the main-loop step (both stage kernels, the ghost fills, the dt kernel), "
+                                "replayed from hipGraphs; 200 algorithmic bytes per cell-update",
+                       "bound": "hbm", "achieved": step_achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                       "frac": step_achieved / HBM_PEAK,
+                       "traffic": traffic["step_bytes"] if traffic else None,
+                       "traffic_source": traffic["source"] if traffic else None,
+                       "alg_bytes_per_step": B_STEP_FP64 * cells,
+                       "kernel": {"name": kname, "achieved": kernel_alg / k2 / 1e9, "unit": "GB/s",
+                                  "frac": kernel_alg / k2 / HBM_PEAK, "alg_bytes_per_launch": kernel_alg,
+                                  "avg_launch_ms": per[3], "traffic": traffic["kernel_bytes"] if traffic else None,
+                                  "timing": "HIP events on the library's stream around each launch",
+                                  "launches_timed": nprof},
+                       "stage_kernel_avg_ms": {"stage1": per[0] + per[1], "stage2": per[2] + per[3]}}
     if not args.no_mixed:
-        log(f"mixed form: {args.steps} steps ...")
-        vm, msm, perm, _, fbm = timed_run(prm, ic, nv.BX_PREC_MIXED, args.steps, args.warmup)
-        out["mixed"] = {"value": vm, "unit": "cell-updates/s", "dtype": "f32 storage+reconstruction, f64 update",
-                        "speedup_vs_fp64": vm / value, "ms_per_step": msm,
-                        "step_hbm_roofline_frac": vm * B_STEP_MIXED / HBM_PEAK,
-                        "k_xz2_hbm_frac": B_XZ2_MIXED * cells / (perm[3] * 1e-3) / HBM_PEAK,
-                        "kernel_avg_ms": {"k_ydir<1>": perm[0], "k_xz<1>": perm[1], "k_ydir<2>": perm[2],
-                                          "k_xz<2>": perm[3]}, "fallback_steps": fbm}
+        msz = args.mixed_size
+        log(f"mixed form at {msz}^3: {args.mixed_steps} steps each precision ...")
+        mprm = box_params(msz)
+        mic = ic if msz == args.size else ics.box_ic(mprm, seed=0)
+        out["mixed"] = mixed_study(mprm, mic, args.mixed_steps)
+        del mic
     if not args.no_graph:
         log("graph replays ...")
         out["graph"] = graph_cost(600, args.steps, args.size)

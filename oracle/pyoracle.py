@@ -182,14 +182,16 @@ def have_reference() -> bool:
 
 def run_reference(prm, ic: Dict[str, np.ndarray], maxsteps: int, mode: int = 0,
                   threads: int = 4, diag: int = 0, workdir: Optional[str] = None,
-                  celldump: int = 0, grav: int = 0) -> Dict[str, np.ndarray]:
+                  celldump: int = 0, grav: int = 0, env_extra: Optional[Dict[str, str]] = None
+                  ) -> Dict[str, np.ndarray]:
     """Run the reference's own subroutines (oracle/_ref/ref_harness) on ``ic``.
 
     Returns dt/time per step and the final cons/u/p; with mode=1 also the
     snapshots after every sub-call of step 1; with celldump=1 (binary) or 2
     (ASCII) also the bytes of the celldump that write_files writes after the
     run (``res["celldump"]``); grav=1 (a fixed potential ic["phi"], calc_gforce)
-    or 2 (and external_grav) turn on the gravity calls of the main loop."""
+    or 2 (and external_grav) turn on the gravity calls of the main loop;
+    env_extra adds environment variables (OpenMP binding) for the run."""
     if not have_reference():
         raise RuntimeError("reference driver not built (make -C oracle ref)")
     from boxzyhydro_amd.ics import write_ic_file
@@ -200,6 +202,7 @@ def run_reference(prm, ic: Dict[str, np.ndarray], maxsteps: int, mode: int = 0,
             f.write(prm.to_namelist())
         write_ic_file(os.path.join(td, "ic.bin"), ic)
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        env.update(env_extra or {})
         if grav and "phi" not in ic:
             raise ValueError("grav needs ic['phi']")
         subprocess.run([REF_HARNESS, "run.nml", "ic.bin", "out", str(maxsteps), str(mode), str(diag),
