@@ -5,24 +5,24 @@
 // TY+4 waves; wave w holds row r = w-2 of the tile (rows -2..TY+1, the two
 // outer rows on each side being the y-halo).  The primitives of a plane are
 // kept in a tile in LDS with one column of x-halo on each side; there are two
-// such tiles, for planes k and k+1.  Per z-plane k:
+// such tiles, for planes k and k+1.  Every row writes its primitives one
+// plane ahead, so that the tile of plane k is complete when plane k starts.
+// Per z-plane k:
 //
 //   A  owner rows 0..TY-1 take the z face k+1/2 from their register queue in
 //      z (the primitives of plane k from the tile, k+1 and k+2 in registers)
-//      and park the z flux difference of plane k in LDS; they write the
-//      primitives of plane k+1 into the other tile.  The halo rows write
-//      their primitives of plane k.  Wave r=TY+1 starts the x-edges of the
-//      owner rows on plane k+1: columns -2,-1,0 and 62..65, seven lanes per
-//      row, their primitives (columns -1 and 64 go into the plane-(k+1)
-//      tile).                                                     -- barrier
+//      and park the z flux difference of plane k in LDS; every row writes its
+//      primitives of plane k+1 into the other tile.  Wave r=TY+1 also takes
+//      the x-edges of the owner rows on plane k+1: columns -2,-1,0 and
+//      62..65, seven lanes per row, their primitives (columns -1 and 64 go
+//      into the plane-(k+1) tile) and x-states.
 //   B  owner rows form their x-states from the neighbours' primitives
 //      (flux.f90:253-351, 391-479), the x-faces l-1/2 (the left plus state
 //      from the next lower lane, or column -1's for lane 0) and with face
 //      l+1/2 (next higher lane, or face 63+1/2) the x flux difference, parked
 //      in LDS; then their y-states, publishing the y-plus state (row r-1's
 //      is needed by row r's face r-1/2).  Row -1 publishes its y-plus state,
-//      row TY keeps its y-minus state.  Wave r=TY+1 forms the x-states of
-//      its edge lanes.                                            -- barrier
+//      row TY keeps its y-minus state.                            -- barrier
 //   C  owner rows and row TY evaluate their face y r-1/2 and publish it over
 //      the y-plus state they read (no other row reads that slot).  Wave
 //      r=TY+1 publishes the x-plus state of column -1 and evaluates face
@@ -31,12 +31,15 @@
 //      everything pointwise up to the next stage (as the two-kernel form
 //      does).
 //
-// Every wave runs a loop of its own role with the same three barriers per
-// plane, so that no register holds a value of another role.  Each face and
-// each cell's primitives are evaluated once per workgroup, except at the
-// tile's edges: the y-halo costs 4 rows of primitives, 2 rows of states and
-// 1 row of faces per TY rows; the x-edges 7 cells of primitives, states and
-// faces per row (one wave pass).
+// A and B need no barrier between them: the tile they write in A is read
+// from the next plane on, and the y-plus slot row r writes in B is read in
+// D only by row r itself.  Every wave runs a loop of its own role with the
+// same two barriers per plane (and one before the first), so that no
+// register holds a value of another role.  Each face and each cell's
+// primitives are evaluated once per workgroup, except at the tile's edges:
+// the y-halo costs 4 rows of primitives, 2 rows of states and 1 row of faces
+// per TY rows; the x-edges 7 cells of primitives, states and faces per row
+// (one wave pass).
 // Every arithmetic operation is the one the exact sequence performs on the
 // same values, so the results are bit-identical to it.
 #pragma once
@@ -174,14 +177,18 @@ __device__ __forceinline__ void tile_owner(const StageArgs &a, TileLds<S, TY> &L
         q2 = t3;
         nxt = load_raw<STAGE, S>(a, zplane(kb + 2), coff);
     } else {
-        nxt = load_raw<STAGE, S>(a, zplane(kb), coff);
+        const W6<S> w = load_prims<STAGE, S>(a, zplane(kb), coff, flg);
+        cs1 = sound_of(w, a.gam);
+        store_w(L.P[kb & 1], c, w);
+        if (kb + 1 < ke) nxt = load_raw<STAGE, S>(a, zplane(kb + 1), coff);
     }
+    __syncthreads();  // every row's primitives of plane kb are in the tile
 
     for (int k = kb; k < ke; k++) {
         const bool ghost_k = xy_ghost || zghost(k);
         const int t = k & 1;
         // A: the z face k+1/2 and the z flux difference of plane k; plane
-        // k+1 into the other tile (flat z: plane k into this tile)
+        // k+1 into the other tile
         if (!a.flatz) {
             const W6<S> q1 = q2;
             cs0 = cs1;
@@ -200,14 +207,16 @@ __device__ __forceinline__ void tile_owner(const StageArgs &a, TileLds<S, TY> &L
             zp_prev = zpl1;
             if (k + 1 < ke) store_w(L.P[t ^ 1], c, q1);
         } else {
-            const Raw<S> cur = nxt;
-            if (k + 1 < ke) nxt = load_raw<STAGE, S>(a, zplane(k + 1), coff);
-            const W6<S> w = prims<STAGE, S>(a, cur, flg);
-            cs0 = sound_of(w, a.gam);
-            store_w(L.P[t], c, w);
+            cs0 = cs1;
+            if (k + 1 < ke) {
+                const Raw<S> cur = nxt;
+                if (k + 2 < ke) nxt = load_raw<STAGE, S>(a, zplane(k + 2), coff);
+                const W6<S> w = prims<STAGE, S>(a, cur, flg);
+                cs1 = sound_of(w, a.gam);
+                store_w(L.P[t ^ 1], c, w);
+            }
         }
-        __syncthreads();
-        // B: x-states, x-faces and the x flux difference; y-states
+        // B (no barrier: plane k's tile was complete before the last one): x-states, x-faces and the x flux difference; y-states
         {
             const W6<S> q0 = fetch_w(L.P[t], c);
             Side<S> xmi, xpl;
@@ -278,16 +287,22 @@ __device__ __forceinline__ void tile_halo(const StageArgs &a, TileLds<S, TY> &L,
     auto zghost = [&](int k) -> bool { return !a.flatz && (k < 2 || k > nz - 3); };
     const int c = (r + 2) * RS + lane + 1;
     const int kb = g.kb, ke = g.ke;
-    Raw<S> nxt = load_raw<STAGE, S>(a, zplane(kb), coff);
+    Raw<S> nxt;
+    store_w(L.P[kb & 1], c, load_prims<STAGE, S>(a, zplane(kb), coff, flg));
+    if (kb + 1 < ke) nxt = load_raw<STAGE, S>(a, zplane(kb + 1), coff);
+    __syncthreads();  // as the owners' first barrier
     for (int k = kb; k < ke; k++) {
         const int t = k & 1;
-        const Raw<S> cur = nxt;
-        if (k + 1 < ke) nxt = load_raw<STAGE, S>(a, zplane(k + 1), coff);
-        const W6<S> w = prims<STAGE, S>(a, cur, flg);
-        store_w(L.P[t], c, w);
-        __syncthreads();
+        if (k + 1 < ke) {
+            // A: primitives of plane k+1 into the other tile
+            const Raw<S> cur = nxt;
+            if (k + 2 < ke) nxt = load_raw<STAGE, S>(a, zplane(k + 2), coff);
+            store_w(L.P[t ^ 1], c, prims<STAGE, S>(a, cur, flg));
+        }
         Side<S> ymi;
         if (r != -2) {
+            // B: y-states of plane k
+            const W6<S> w = fetch_w(L.P[t], c);
             Side<S> ypl;
             dir_states<1>(fetch_w(L.P[t], c - RS), w, fetch_w(L.P[t], c + RS), sound_of(w, a.gam),
                           xy_ghost || zghost(k), a, ymi, ypl);
@@ -358,7 +373,9 @@ __device__ __forceinline__ void tile_edge(const StageArgs &a, TileLds<S, TY> &L,
     };
 
     const int kb = g.kb, ke = g.ke;
-    Raw<S> nxt = load_raw<STAGE, S>(a, zplane(kb), coff);
+    Raw<S> nxt;
+    store_w(L.P[kb & 1], c, load_prims<STAGE, S>(a, zplane(kb), coff, flg));
+    if (kb + 1 < ke) nxt = load_raw<STAGE, S>(a, zplane(kb + 1), coff);
     Raw<S> enxt{};
     if (e_on) {
         edge_prims(kb, load_raw<STAGE, S>(a, zplane(kb), eoff));
@@ -366,21 +383,21 @@ __device__ __forceinline__ void tile_edge(const StageArgs &a, TileLds<S, TY> &L,
         edge_publish();
         if (kb + 1 < ke) enxt = load_raw<STAGE, S>(a, zplane(kb + 1), eoff);
     }
+    __syncthreads();  // as the owners' first barrier
     for (int k = kb; k < ke; k++) {
         const bool more = k + 1 < ke;
-        {
+        if (more) {
+            // A: primitives of plane k+1, its own row and the x-edges
             const Raw<S> cur = nxt;
-            if (more) nxt = load_raw<STAGE, S>(a, zplane(k + 1), coff);
-            store_w(L.P[k & 1], c, prims<STAGE, S>(a, cur, flg));
+            if (k + 2 < ke) nxt = load_raw<STAGE, S>(a, zplane(k + 2), coff);
+            store_w(L.P[(k + 1) & 1], c, prims<STAGE, S>(a, cur, flg));
+            if (e_on) {
+                const Raw<S> ecur = enxt;
+                if (k + 2 < ke) enxt = load_raw<STAGE, S>(a, zplane(k + 2), eoff);
+                edge_prims(k + 1, ecur);
+                edge_states(k + 1);  // B
+            }
         }
-        if (e_on && more) {
-            // A: primitives of plane k+1 (its tile is not read before the next B)
-            const Raw<S> cur = enxt;
-            if (k + 2 < ke) enxt = load_raw<STAGE, S>(a, zplane(k + 2), eoff);
-            edge_prims(k + 1, cur);
-        }
-        __syncthreads();
-        if (e_on && more) edge_states(k + 1);  // B
         __syncthreads();
         if (e_on && more) edge_publish();  // C: the owners read plane k's in B
         __syncthreads();
