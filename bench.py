@@ -393,6 +393,31 @@ def adjoint_cost(n: int, nsteps: int, every: int):
 B_STATE_TAB = 64.0  # tabulated state(): read rho, E, u (5 x 8 B), write p, adindx, muc_array (3 x 8 B)
 
 
+def walls_cost(n: int, steps: int, warmup: int, outflow_ms: float):
+    """The headline box with walls on every side, the reference's shipped
+    in.nml (utils.f90:338-403): the same graph-replayed fused step, whose
+    ghost fill also writes the 16 two-level corner ghosts; ms per step
+    against the outflow box's."""
+    import dataclasses
+
+    from boxzyhydro_amd import ics
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+    from boxzyhydro_amd.params import box_params
+
+    prm = dataclasses.replace(box_params(n), no_outflow_xl=True, no_outflow_xr=True, no_outflow_yl=True,
+                              no_outflow_yr=True, no_outflow_zl=True, no_outflow_zr=True)
+    ic = ics.box_ic(prm, seed=0)
+    value, ms, _, _, fb = timed_run(prm, ic, nv.BX_PREC_FP64, steps, warmup)
+    g = HydroGrid(prm, ic, device=0)
+    g.run(3, mode=nv.BX_STEP_FUSED, endtime=1e300)
+    fused, exact = g.step_counts()
+    del g
+    return {"workload": f"3-D box {n}^3 interior FP64, walls on all six sides, otherwise the headline box",
+            "cell_updates_per_s": value, "ms_per_step": ms, "over_outflow": ms / outflow_ms,
+            "fallback_steps": fb, "path_of_3_fused_mode_steps": {"fused": fused, "exact": exact}}
+
+
 def eos_cost(n: int, steps: int, reps: int):
     """The tabulated H2 EOS (H2STAT=0, the reference's default 10000 x 1000
     tables) on an n^3 molecular-cloud box: host table build, the state()
@@ -526,6 +551,7 @@ def main() -> int:
                     help="batched boxes without the one-grid-at-a-time comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost")
     ap.add_argument("--no-eos", action="store_true", help="skip the tabulated-EOS line")
+    ap.add_argument("--no-walls", action="store_true", help="skip the all-walls box line")
     ap.add_argument("--no-options", action="store_true", help="skip the flux_ang / gravity line")
     ap.add_argument("--eos-only", action="store_true", help="run only the tabulated-EOS line")
     ap.add_argument("--options-only", action="store_true", help="run only the flux_ang / gravity line")
@@ -645,6 +671,9 @@ def main() -> int:
                                   "timing": "HIP events on the library's stream around each launch",
                                   "launches_timed": nprof},
                        "stage_kernel_avg_ms": {"stage1": per[0] + per[1], "stage2": per[2] + per[3]}}
+    if not args.no_walls:
+        log("all-walls box ...")
+        out["walls"] = walls_cost(args.size, args.steps, args.warmup, ms_step)
     if not args.no_mixed:
         msz = args.mixed_size
         log(f"mixed form at {msz}^3: {args.mixed_steps} steps each precision ...")

@@ -827,6 +827,9 @@ extern "C" int bx_adjoint_forward(bx_ctx *c, int32_t nsteps, int32_t every, cons
     if (c->eos) return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: fixed-gamma EOS (H2STAT=-1) only");
     if (c->prm.fluxangmom) return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: fluxangmom off only");
     if (c->grav_flags) return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: no gravity configuration");
+    if (c->nd2 > 0)
+        return set_error(BX_ERR_UNSUPPORTED, "bx_adjoint_forward: two-level corner ghosts (walls in y and z) are "
+                                             "not differentiated");
     BX_HIP(hipSetDevice(c->device));
     int rc;
     if (!c->fused_ready) {

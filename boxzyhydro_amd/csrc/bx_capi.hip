@@ -256,24 +256,58 @@ int bx_create(const bx_params *prm, int32_t device, bx_ctx **out)
     c->nscratch = (int64_t)gx.size();
     // The pair of calls composed: a ghost whose donor is the pre-call value of
     // ghost h gets h's value after the first call, which is h's donor when
-    // that is an interior cell (sign flips compose).  Otherwise the pair
-    // leaves a value from before it and the fused step cannot apply.
-    std::vector<int32_t> fs(gs), fg(gg);
+    // that is an interior cell (sign flips compose).  When h's donor is the
+    // pre-call value of ghost h2 the pair leaves h2's value from before it;
+    // the fused step supports that when h2's own pair gives an interior cell
+    // v (the corner ghosts of a box walled in y and z): such a ghost ends a
+    // step as +-(v's corrector result before source(1)) and is fixed up after
+    // the ghost fill.  Deeper chains take the exact sequence.
+    std::vector<int32_t> fs(gs), fg(gg), d2dst, d2src, d2sign;
     c->fghost_ok = true;
     if (!gx.empty()) {
         std::vector<int64_t> entry((size_t)n, -1);
         for (size_t k = 0; k < gd.size(); k++) entry[(size_t)gd[k]] = (int64_t)k;
-        for (size_t k = 0; k < gd.size() && c->fghost_ok; k++) {
-            if (gs[k] >= 0) continue;
+        // the pair's value of entry k: -1 with the interior cell and sign, or
+        // the entry of the ghost whose pre-pair value it is (sign: its sign)
+        auto pair_of = [&](size_t k, int32_t &cell, int32_t &sgn) -> int64_t {
+            if (gs[k] >= 0) {
+                cell = gs[k];
+                sgn = gg[k];
+                return -1;
+            }
             const int64_t h = entry[(size_t)gx[(size_t)(-gs[k] - 1)]];
-            if (h < 0 || gs[(size_t)h] < 0) {
+            if (h < 0) return -2;
+            sgn = gg[k] ^ gg[(size_t)h];
+            if (gs[(size_t)h] >= 0) {
+                cell = gs[(size_t)h];
+                return -1;
+            }
+            const int64_t h2 = entry[(size_t)gx[(size_t)(-gs[(size_t)h] - 1)]];
+            return h2 < 0 ? -2 : h2;
+        };
+        for (size_t k = 0; k < gd.size() && c->fghost_ok; k++) {
+            int32_t cell = 0, sgn = 0;
+            const int64_t r = pair_of(k, cell, sgn);
+            if (r == -1) {
+                fs[k] = cell;
+                fg[k] = sgn;
+                continue;
+            }
+            int32_t cell2 = 0, sgn2 = 0;
+            if (r < 0 || pair_of((size_t)r, cell2, sgn2) != -1) {
                 c->fghost_ok = false;
                 break;
             }
-            fs[k] = gs[(size_t)h];
-            fg[k] = gg[k] ^ gg[(size_t)h];
+            fs[k] = cell2;
+            fg[k] = sgn ^ sgn2;
+            d2dst.push_back(gd[k]);
+            d2src.push_back(cell2);
+            d2sign.push_back(sgn ^ sgn2);
         }
+        if ((int)d2dst.size() > kMaxD2) c->fghost_ok = false;
+        if (!c->fghost_ok) d2dst.clear();
     }
+    c->nd2 = (int)d2dst.size();
 
     auto fail = [&](hipError_t e, const char *what) {
         int r = hip_check(e, what);
@@ -316,6 +350,20 @@ int bx_create(const bx_params *prm, int32_t device, bx_ctx **out)
             hipSuccess)
             return fail(e, "hipMemcpy ghosts");
     }
+    if (c->nd2 > 0) {
+        const size_t b = sizeof(int32_t) * (size_t)c->nd2;
+        if ((e = hipMalloc((void **)&c->d2_dst, b)) != hipSuccess) return fail(e, "hipMalloc corner ghosts");
+        if ((e = hipMalloc((void **)&c->d2_src, b)) != hipSuccess) return fail(e, "hipMalloc corner ghosts");
+        if ((e = hipMalloc((void **)&c->d2_sign, b)) != hipSuccess) return fail(e, "hipMalloc corner ghosts");
+        if ((e = hipMalloc((void **)&c->d2_ubar, sizeof(double) * 6 * (size_t)c->nd2)) != hipSuccess)
+            return fail(e, "hipMalloc corner ghosts");
+        if ((e = hipMemcpy(c->d2_dst, d2dst.data(), b, hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "hipMemcpy corner ghosts");
+        if ((e = hipMemcpy(c->d2_src, d2src.data(), b, hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "hipMemcpy corner ghosts");
+        if ((e = hipMemcpy(c->d2_sign, d2sign.data(), b, hipMemcpyHostToDevice)) != hipSuccess)
+            return fail(e, "hipMemcpy corner ghosts");
+    }
     if (!gx.empty()) {
         if ((e = hipMalloc((void **)&c->ghost_scratch_src, sizeof(int32_t) * gx.size())) != hipSuccess)
             return fail(e, "hipMalloc ghost scratch");
@@ -346,7 +394,7 @@ int bx_destroy(bx_ctx *c)
     drop_step_graphs(c);
     void *dev[] = {c->cons, c->cons_old, c->u, c->p, c->adindx, c->qq, c->states, c->ghost_dst, c->ghost_src,
                    c->ghost_sign, c->ghost_scratch_src, c->ghost_scratch, c->fghost_src, c->fghost_sign,
-                   c->ustar, c->pstar, c->unext, c->dsc,
+                   c->d2_dst, c->d2_src, c->d2_sign, c->d2_ubar, c->ustar, c->pstar, c->unext, c->dsc,
                    c->cons32, c->ustar32, c->unext32, c->pstar32, c->eflux, c->chunk0, c->chunk_dts, c->stage};
     for (void *p : dev)
         if (p) (void)hipFree(p);

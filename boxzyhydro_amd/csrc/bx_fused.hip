@@ -152,6 +152,9 @@ StageArgs base_args(bx_ctx *c, double dt)
     a.srho = c->prm.small_rho;
     a.vlimit = c->prm.vlimit;
     a.flags = &c->dsc->flags;
+    a.nd2 = c->nd2;
+    a.d2cell = c->d2_src;
+    a.d2ubar = c->d2_ubar;
     return a;
 }
 
@@ -204,11 +207,24 @@ static int launch_stage(bx_ctx *c, const StageArgs &a, hipEvent_t mid, S *eflux)
     return BX_OK;
 }
 
+// the pressure plane of the materialised state at the corner ghosts (the
+// value the last fused step's fix-up recorded)
+__global__ __launch_bounds__(64) void k_corner_pressure(int nd2, const int32_t *__restrict__ dst,
+                                                        const double *__restrict__ ubar, double *__restrict__ p)
+{
+    const int j = threadIdx.x;
+    if (j < nd2) p[dst[j]] = ubar[6 * j + 5];
+}
+
 int launch_materialize(bx_ctx *c)
 {
     hipLaunchKernelGGL(k_materialize, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cons, c->u, c->p,
                        c->adindx, c->prm.gammafix, c->prm.small_eps);
     BX_HIP(hipGetLastError());
+    if (c->nd2 > 0) {
+        hipLaunchKernelGGL(k_corner_pressure, dim3(1), dim3(64), 0, c->stream, c->nd2, c->d2_dst, c->d2_ubar, c->p);
+        BX_HIP(hipGetLastError());
+    }
     return BX_OK;
 }
 
@@ -217,7 +233,9 @@ int fused_supported(const bx_ctx *c)
 {
     if (c->nscratch > 0 && !c->fghost_ok)
         return set_error(BX_ERR_UNSUPPORTED, "fused step needs the ghost donors of a pair of set_ghost_cells calls "
-                                             "to be interior cells");
+                                             "to resolve to interior cells within two levels");
+    if (c->nd2 > 0 && c->prm.precision != BX_PREC_FP64)
+        return set_error(BX_ERR_UNSUPPORTED, "fused step: two-level corner ghosts in the FP64 form only");
     if (c->grav[0] != 0.0 || c->grav[1] != 0.0 || c->grav[2] != 0.0)
         return set_error(BX_ERR_UNSUPPORTED, "fused step assumes zero gravity");
     if (c->prm.fluxangmom) return set_error(BX_ERR_UNSUPPORTED, "fused step fluxes linear momentum (fluxangmom off)");
@@ -302,6 +320,110 @@ static int ghosts6(bx_ctx *c, S *planes, S *pstar)
     return BX_OK;
 }
 
+// The two-level corner ghosts after the corrector's ghost fill.  The exact
+// sequence leaves such a ghost holding +-(its interior cell v's corrector
+// result before source(1)); that value reaches the results only through the
+// next step's Courant rate (courant.f90:11-15 runs over every cell) and the
+// stored state.  One lane per ghost evaluates v's corrector update again
+// from the corrector's inputs with the tiled kernel's operations (its six
+// faces, flux.f90:121-190, and the update of flux.f90:213-228), parks the
+// result before source(1), writes the ghost and adds its Courant rate (the
+// state and velocity of main.f90:232 and of the step's last velocity call,
+// state.f90:30-57, velocity.f90:18-21).
+__global__ __launch_bounds__(64) void k_corner_ghosts(StageArgs a0, const int32_t *__restrict__ dst,
+                                                      const int32_t *__restrict__ sign)
+{
+    const StageArgs a = with_dt(a0);
+    const int j = threadIdx.x;
+    double rt = 0.0;
+    unsigned int flg = 0;
+    if (j < a.nd2) {
+        const int64_t v = a.d2cell[j];
+        const int64_t plane = (int64_t)a.nx * a.ny;
+        const int ix = (int)(v % a.nx), iy = (int)((v / a.nx) % a.ny), iz = (int)(v / plane);
+        auto ghost = [&](int x, int y, int z) {
+            return x < 2 || x > a.nx - 3 || y < 2 || y > a.ny - 3 || (!a.flatz && (z < 2 || z > a.nz - 3));
+        };
+        double F[3][2][5];
+        for (int d = 0; d < 3; d++)
+            for (int h = 0; h < 2; h++)
+                for (int i = 0; i < 5; i++) F[d][h][i] = 0.0;
+        W6<double> q0{};
+        auto faces = [&](auto dirc) {
+            constexpr int d = decltype(dirc)::value;
+            const int64_t e = d == 0 ? 1 : (d == 1 ? a.nx : plane);
+            W6<double> w[5];
+            for (int i = 0; i < 5; i++) w[i] = load_prims<2, double>(a, v + (i - 2) * e, 0u, flg);
+            if (d == 0) q0 = w[2];
+            Side<double> mi[3], pl[3];
+            for (int i = 1; i <= 3; i++) {
+                const int o = i - 2;
+                const bool gh = ghost(ix + (d == 0 ? o : 0), iy + (d == 1 ? o : 0), iz + (d == 2 ? o : 0));
+                dir_states<d>(w[i - 1], w[i], w[i + 1], sound_of(w[i], a.gam), gh, a, mi[i - 1], pl[i - 1]);
+            }
+            face_flux<double, d>(pl[0], mi[1], F[d][0]);
+            face_flux<double, d>(pl[1], mi[2], F[d][1]);
+        };
+        faces(std::integral_constant<int, 0>{});
+        faces(std::integral_constant<int, 1>{});
+        if (!a.flatz) faces(std::integral_constant<int, 2>{});
+        double ui[5], uo[5], ft[5];
+        for (int i = 0; i < 5; i++) {
+            ui[i] = elem<double>(a.uin, (int64_t)i * a.n + v, 0u);
+            uo[i] = elem<double>(a.uold, (int64_t)i * a.n + v, 0u);
+            const double dx = F[0][1][i] - F[0][0][i];
+            const double ey = a.axz * (F[1][1][i] - F[1][0][i]);
+            const double dz = a.flatz ? 0.0 : F[2][1][i] - F[2][0][i];
+            const double sm = (a.ayz * dx + ey) + a.axy * dz;
+            ft[i] = -div_by(sm * a.dt, a.vol, a.inv_vol, a.vol_pow2);
+        }
+        double rv = 0.0;
+        // rewrites U^{n+1} of v with the bits the tiled kernel wrote and parks
+        // the result before source(1) in d2ubar
+        stage_update<2, double>(a, ui, uo, ft, q0, v, 0u, flg, rv, true);
+        double U[5];
+        for (int i = 0; i < 5; i++) U[i] = a.d2ubar[6 * j + i];
+        // the pressure the exact sequence leaves in the ghost: the state call
+        // after the last ghost pair (main.f90:511-513) takes the velocity of
+        // the previous velocity call, when the ghost held h2's value from
+        // before the third pair, 0.5*(cons_old + U* + 0) of v up to signs
+        // (flux.f90:213, cons_old = U^n after source(0), which leaves rho and
+        // m as they are); the signs drop out of |u|^2
+        {
+            double rc = 0.5 * ((uo[0] + ui[0]) + 0.0);
+            rc = vmax(rc, a.srho);
+            double uc[3];
+            for (int d = 0; d < 3; d++) uc[d] = (0.5 * ((uo[1 + d] + ui[1 + d]) + 0.0)) / rc;
+            const double ek = U[0] * 0.5 * (uc[0] * uc[0] + uc[1] * uc[1] + uc[2] * uc[2]);
+            a.d2ubar[6 * j + 5] = vmax(U[4] - ek, a.seps) * a.gm1;
+        }
+        const int sg = sign[j];
+        for (int f = 1; f <= 3; f++)
+            if ((sg >> (f - 1)) & 1) U[f] = -U[f];
+        const int64_t gcell = dst[j];
+        double *un = (double *)a.uout;
+        for (int i = 0; i < 5; i++) un[(int64_t)i * a.n + gcell] = U[i];
+        const double v0 = U[1] / U[0], v1 = U[2] / U[0], v2 = U[3] / U[0];
+        const double ekin = U[0] * 0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
+        const double p = vmax(U[4] - ekin, a.seps) * a.gm1;
+        const double snd = sqrt(a.gam * p / U[0]);
+        rt = div_by(snd + fabs(v0), a.ds[0], a.inv_ds[0], a.ds_pow2[0]) +
+             div_by(snd + fabs(v1), a.ds[1], a.inv_ds[1], a.ds_pow2[1]) +
+             div_by(snd + fabs(v2), a.ds[2], a.inv_ds[2], a.ds_pow2[2]);
+    }
+    if (flg) atomicOr(a.flags, flg);
+    wave_max_to(rt, a.rate);
+}
+
+// after the corrector's ghost fill of un (b: the corrector's arguments)
+static int corner_ghosts(bx_ctx *c, const StageArgs &b)
+{
+    if (c->nd2 == 0) return BX_OK;
+    hipLaunchKernelGGL(k_corner_ghosts, dim3(1), dim3(64), 0, c->stream, b, c->d2_dst, c->d2_sign);
+    BX_HIP(hipGetLastError());
+    return BX_OK;
+}
+
 int narrow_cons(bx_ctx *c)
 {
     hipLaunchKernelGGL(k_narrow, dim3(blocks_for(5 * c->n, 256)), dim3(256), 0, c->stream, 5 * c->n, c->cons,
@@ -349,6 +471,7 @@ static int fused_step_t(bx_ctx *c, double dt, S *u0, S *us, S *ps, S *un, bool r
     if ((rc = launch_stage<2, S>(c, b, c->prof_on ? c->ev[4] : nullptr, (S *)c->eflux))) return rc;
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[5], c->stream));
     if ((rc = ghosts6<S>(c, un, (S *)nullptr))) return rc;
+    if constexpr (sizeof(S) == 8) return corner_ghosts(c, b);
     return BX_OK;
 }
 
@@ -413,6 +536,8 @@ static int enqueue_pair(bx_ctx *c, S *u0, S *us, S *ps, S *un)
         b.rate = &c->dsc->rate_bits[0];
         if ((rc = launch_stage<2, S>(c, b, nullptr, (S *)c->eflux))) return rc;
         if ((rc = ghosts6<S>(c, dst, (S *)nullptr))) return rc;
+        if constexpr (sizeof(S) == 8)
+            if ((rc = corner_ghosts(c, b))) return rc;
     }
     return BX_OK;
 }
@@ -515,7 +640,8 @@ int fused_step_words(bx_ctx *c, double dt, const double *u0, double *us, double 
     b.uout = un;
     b.rate = rate;
     if ((rc = launch_stage<2, double>(c, b, nullptr, ey))) return rc;
-    return ghosts6<double>(c, un, (double *)nullptr);
+    if ((rc = ghosts6<double>(c, un, (double *)nullptr))) return rc;
+    return corner_ghosts(c, b);
 }
 
 // A step whose dt is formed on the device from the Courant rate of its state

@@ -11,6 +11,8 @@
 
 namespace bx {
 
+constexpr int kMaxD2 = 64;  // two-level ghost compositions a context may have (one wave)
+
 // Scalars that live on the device so whole steps can run without host syncs.
 struct DevScalars {
     unsigned long long rate_bits[2];  // Courant max-rate (non-negative doubles as bits), ping-pong
@@ -74,6 +76,18 @@ struct bx_ctx {
     int32_t *fghost_src;
     int32_t *fghost_sign;
     bool fghost_ok;
+    // ghosts whose pair leaves the pre-pair value of another ghost h2 (whose
+    // own pair gives an interior cell v): the corner ghosts of a box walled
+    // in y and z.  After a step such a ghost holds +-(v's corrector result
+    // before source(1)); a fix-up kernel after the corrector's ghost fill
+    // writes it (d2_ubar: 6 doubles per entry, its scratch, the last one the
+    // ghost's pressure for the materialised planes).  fghost_src/sign map
+    // them to v for the other fills.
+    int nd2;
+    int32_t *d2_dst;   // ghost cell
+    int32_t *d2_src;   // interior cell v
+    int32_t *d2_sign;  // sign bits of the composition
+    double *d2_ubar;
 
     // fused-path state
     double *ustar;     // 5 planes: stage-1 result U*

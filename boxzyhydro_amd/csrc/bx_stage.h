@@ -45,6 +45,12 @@ struct StageArgs {
     unsigned int *flags;
     unsigned long long *rate;  // stage 2: next step's Courant rate (max)
     const double *dtp;         // when set, the step's dt is read from here (graph replays), else dt
+    // the donors of the two-level corner ghosts, whose corrector result
+    // before source(1) is parked in d2ubar (6 per entry: that, then the
+    // ghost's pressure)
+    int nd2;
+    const int32_t *d2cell;
+    double *d2ubar;
 };
 
 // The arguments of a launch with dt resolved: a kernel reads it once at entry.

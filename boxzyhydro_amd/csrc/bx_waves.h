@@ -91,7 +91,7 @@ __device__ __forceinline__ void ydir_wave(const StageArgs &a, S *__restrict__ Ey
 template <int STAGE, typename S>
 __device__ __forceinline__ void stage_update(const StageArgs &a, const double *ui, const double *uo, const double *ft,
                                              const W6<S> &q0, int64_t c, unsigned coff, unsigned int &flg,
-                                             double &rmax)
+                                             double &rmax, bool park = false)
 {
     const int64_t n = a.n;
     double U[5];
@@ -117,6 +117,15 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, const double *u
     if (!(U[0] > a.srho) || U[4] < 0.0) flg |= FLAG_FLOOR;
     U[0] = vmax(U[0], a.srho);
     if (U[4] < 0.0) U[4] = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0] + a.seps;
+    if constexpr (STAGE == 2 && sizeof(S) == 8) {
+        // the donor of a two-level corner ghost: park the result before source(1)
+        if (park) {
+            const int64_t cell = c + (int64_t)(coff / sizeof(S));
+            for (int j = 0; j < a.nd2; j++)
+                if (a.d2cell[j] == cell)
+                    for (int i = 0; i < 5; i++) a.d2ubar[6 * j + i] = U[i];
+        }
+    }
     if (STAGE == 1) {
         // p* = state() at main.f90:353 on U* with the velocity of :318
         if (U[0] < 1e-4) flg |= FLAG_LOWRHO;

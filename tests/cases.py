@@ -46,6 +46,25 @@ def stress_ic(prm: HydroParams, seed: int):
     return ic
 
 
+def corner_blob_ic(prm: HydroParams, seed: int, boost: float = 6.0, vboost: float = 2.0):
+    """A seeded box whose fastest cells are the interior cells next to the
+    low-y, low-z edges at both x ends (x in {2,3,nx-4,nx-3}, y and z in
+    {2,3}, 0-based): the donors of the two-level corner ghosts of a box
+    walled in y and z, so that those ghosts' Courant rates compete for dt."""
+    base = ics.box_ic(prm, seed=seed, sigma_cells=2.0)
+    shp = prm.shape  # (nz, ny, nx)
+    rho = base["cons"][:, 0].reshape(shp).copy()
+    v = base["u"].reshape(shp + (3,)).copy()
+    p = base["p"].reshape(shp).copy()
+    xs = [2, 3, prm.nx - 4, prm.nx - 3]
+    for z in (2, 3):
+        for y in (2, 3):
+            for x in xs:
+                p[z, y, x] *= boost
+                v[z, y, x, :] *= vboost
+    return ics._assemble(prm, rho, v[..., 0], v[..., 1], v[..., 2], p)
+
+
 def eos_cells_ic(prm: HydroParams, seed: int):
     """Independent random cells for one call of the tabulated state()
     (state.f90:58-79): densities inside the tables, below them, above them and

@@ -114,10 +114,9 @@ def test_box64_matches_oracle():
 def test_fused_step_matches_reference(golden, name):
     """The fused step (with its exact-sequence fallback) gives the
     reference's dt sequence and final state bit for bit.  The path taken is
-    checked too: box12_walls (walls on every side of a 3-D box, ghost donors
-    chained beyond a pair of set_ghost_cells calls) takes the exact sequence
-    every step; every other case takes the fused kernels after the first
-    step, except for the steps it redoes (stress8 forces irregular cells)."""
+    checked too: every case takes the fused kernels after the first step
+    (box12_walls, walls on every side of a 3-D box, among them), except for
+    the steps it redoes (stress8 forces irregular cells)."""
     from boxzyhydro_amd import native as nv
 
     rec = golden[name]
@@ -133,9 +132,7 @@ def test_fused_step_matches_reference(golden, name):
     fb = g.fallback_count()
     print(f"  fused steps {fused}, exact steps {exact}, fallbacks {fb}", flush=True)
     assert fused + exact == rec["steps"]
-    if name == "box12_walls":
-        assert fused == 0
-    elif name == "stress8":
+    if name == "stress8":
         assert exact >= 1 + fb  # a step from an irregular state is taken exactly without a fused attempt
     else:
         assert exact == 1 and fb == 0
