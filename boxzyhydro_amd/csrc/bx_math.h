@@ -36,6 +36,39 @@ __device__ __forceinline__ T vmax(T a, T b) { return a > b ? a : b; }
 template <typename T>
 __device__ __forceinline__ T vmin(T a, T b) { return a < b ? a : b; }
 
+// vmax(x, c) for a floor c >= +0: x > c ? x : c and the hardware maximum
+// give the same value for every x (a NaN x gives c either way; a zero x of
+// either sign is not above c, and for c = +0 both give +0), so the single
+// instruction is used.
+template <typename T>
+__device__ __forceinline__ T floor_at(T x, T c) { return Arith<T>::fmax(x, c); }
+
+// FP64 square roots as the compiler lowers sqrt() (correctly rounded): for
+// x below 2^-767 it scales x by 2^256 and the root by 2^-128, and it answers
+// x itself for +-0 and +inf; otherwise the lowering is
+//     y = rsq(x); s0 = x*y; h0 = 0.5*y; r = fma(-h0, s0, 0.5)
+//     h1 = fma(h0, r, h0); s1 = fma(s0, r, s0)
+//     s2 = fma(fma(-s1, s1, x), h1, s1); root = fma(fma(-s2, s2, x), h1, s2)
+// which is taken here for x in [2^-767, inf); a lane with another x (zero,
+// negative, infinite, NaN or tiny) takes sqrt() for all N.
+template <int N>
+__device__ __forceinline__ void sqrt_n(const double *x, double *r)
+{
+    bool ok = true;
+    for (int i = 0; i < N; i++) {
+        const double v = x[i];
+        const double y = __builtin_amdgcn_rsq(v);
+        const double s0 = v * y, h0 = y * 0.5;
+        const double rr = __builtin_fma(-h0, s0, 0.5);
+        const double h1 = __builtin_fma(h0, rr, h0), s1 = __builtin_fma(s0, rr, s0);
+        const double s2 = __builtin_fma(__builtin_fma(-s1, s1, v), h1, s1);
+        r[i] = __builtin_fma(__builtin_fma(-s2, s2, v), h1, s2);
+        ok = ok && v >= 0x1p-767 && v < __builtin_huge_val();
+    }
+    if (__builtin_expect(!ok, 0))
+        for (int i = 0; i < N; i++) r[i] = __builtin_sqrt(x[i]);
+}
+
 // utils.f90:696-707
 template <typename T>
 __device__ __forceinline__ T genminmod(T x1, T x2, T x3)
@@ -192,8 +225,17 @@ __device__ __forceinline__ void face_flux(const Side<T> &L, const Side<T> &R, T 
     fr[3] = (dir == 2 ? R.p : zero) + sr[3] * vr;
     fr[4] = (sr[4] + R.p) * vr;
     // flux_hll, flux.f90:583-589
-    const T cr = Arith<T>::root(Arith<T>::div(R.g * R.p, R.d));
-    const T cl = Arith<T>::root(Arith<T>::div(L.g * L.p, L.d));
+    T cr, cl;
+    if constexpr (sizeof(T) == 8) {
+        const double sq[2] = {Arith<T>::div(R.g * R.p, R.d), Arith<T>::div(L.g * L.p, L.d)};
+        double rt[2];
+        sqrt_n<2>(sq, rt);
+        cr = rt[0];
+        cl = rt[1];
+    } else {
+        cr = Arith<T>::root(Arith<T>::div(R.g * R.p, R.d));
+        cl = Arith<T>::root(Arith<T>::div(L.g * L.p, L.d));
+    }
     const T ap = vmax(vmax(zero, vr + cr), vl + cl);
     const T am = vmax(vmax(zero, -(vr - cr)), -(vl - cl));
     const T apam = ap * am;

@@ -106,25 +106,26 @@ __device__ __forceinline__ T comp(const W6<T> &w, int j)
 // The reference forms each state as (q + c*s) + 0 (the transverse terms are
 // exact zeros); that equals (q + 0) + c*s bit for bit, so the centre values
 // are made canonical once (no -0) and a zero slope of either sign gives the
-// same state (slope_fz).
+// same state (slope_fz).  For the same reason a ghost cell zeroes the two
+// coefficients c instead of its six slopes (c*s is then a zero for the
+// finite slopes of finite data).  lp and lm enter only as 1-lp and -1-lm,
+// where a zero of either sign gives the same value, so the hardware maximum
+// and minimum stand for vmax/vmin against 0.
 template <int d, typename T>
 __device__ __forceinline__ void dir_states(const W6<T> &wm, const W6<T> &wc, const W6<T> &wp, T sound, bool ghost,
                                            const StageArgs &a, Side<T> &mi, Side<T> &pl)
 {
-    T su0 = T(0), su1 = T(0), su2 = T(0), sp = T(0), sd = T(0), se = T(0);
-    if (!ghost) {
-        su0 = slope_fz(wm.u0, wc.u0, wp.u0);
-        su1 = slope_fz(wm.u1, wc.u1, wp.u1);
-        su2 = slope_fz(wm.u2, wc.u2, wp.u2);
-        sp = slope_fz(wm.p, wc.p, wp.p);
-        sd = slope_fz(wm.d, wc.d, wp.d);
-        se = slope_fz(wm.e, wc.e, wp.e);
-    }
+    const T su0 = slope_fz(wm.u0, wc.u0, wp.u0);
+    const T su1 = slope_fz(wm.u1, wc.u1, wp.u1);
+    const T su2 = slope_fz(wm.u2, wc.u2, wp.u2);
+    const T sp = slope_fz(wm.p, wc.p, wp.p);
+    const T sd = slope_fz(wm.d, wc.d, wp.d);
+    const T se = slope_fz(wm.e, wc.e, wp.e);
     const T ud = comp(wc, d);
     const T dt = T(a.dt);
-    const T lp = div_by(vmax(ud + sound, T(0)) * dt, a.ds[d], a.inv_ds[d], a.ds_pow2[d]);
-    const T lm = div_by(vmin(ud - sound, T(0)) * dt, a.ds[d], a.inv_ds[d], a.ds_pow2[d]);
-    const T cm = T(0.5) * (T(-1) - lm), cp = T(0.5) * (T(1) - lp);
+    const T lp = div_by(Arith<T>::fmax(ud + sound, T(0)) * dt, a.ds[d], a.inv_ds[d], a.ds_pow2[d]);
+    const T lm = div_by(Arith<T>::fmin(ud - sound, T(0)) * dt, a.ds[d], a.inv_ds[d], a.ds_pow2[d]);
+    const T cm = ghost ? T(0) : T(0.5) * (T(-1) - lm), cp = ghost ? T(0) : T(0.5) * (T(1) - lp);
     const T z = T(0);  // transverse terms: 0*cc with cc=dt*half/ds > 0
     const T d0 = wc.d + z, v0 = wc.u0 + z, v1 = wc.u1 + z, v2 = wc.u2 + z, p0 = wc.p + z, e0 = wc.e + z;
     mi.d = d0 + cm * sd;
@@ -146,7 +147,14 @@ __device__ __forceinline__ void dir_states(const W6<T> &wm, const W6<T> &wc, con
 template <typename T>
 __device__ __forceinline__ T sound_of(const W6<T> &w, double gam)
 {
-    return Arith<T>::root(Arith<T>::div(w.p * T(gam), w.d));
+    if constexpr (sizeof(T) == 8) {
+        const double x = Arith<T>::div(w.p * T(gam), w.d);
+        double r;
+        sqrt_n<1>(&x, &r);
+        return r;
+    } else {
+        return Arith<T>::root(Arith<T>::div(w.p * T(gam), w.d));
+    }
 }
 
 
@@ -198,12 +206,12 @@ __device__ __forceinline__ W6<S> prims(const StageArgs &a, const Raw<S> &v, unsi
             // source(0) energy re-round with g = -0 (source.f90:176-182), then
             // state at main.f90:314 with u = m/rho, velocity at :318
             const double ek = quo[3];
-            const double E1 = vmax(E - ek, a.seps) + ek;
-            w.e = vmax(E1 - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
+            const double E1 = floor_at(E - ek, a.seps) + ek;
+            w.e = floor_at(E1 - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
             w.p = w.e * a.gm1;
         } else {
             // velocity at main.f90:355; p* is the state of :353 (stale velocity)
-            w.e = vmax(E - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
+            w.e = floor_at(E - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
             w.p = v.p;
         }
         if (fabs(w.u0) > a.vlimit || fabs(w.u1) > a.vlimit || fabs(w.u2) > a.vlimit) flg |= FLAG_CLAMP;
@@ -212,7 +220,7 @@ __device__ __forceinline__ W6<S> prims(const StageArgs &a, const Raw<S> &v, unsi
         const double inv = 1.0 / r;
         const double v0 = m0 * inv, v1 = m1 * inv, v2 = m2 * inv;
         const double ek = 0.5 * (m0 * v0 + m1 * v1 + m2 * v2);
-        const double e = vmax(E - ek, a.seps);
+        const double e = floor_at(E - ek, a.seps);
         w.d = (float)r;
         w.u0 = (float)v0;
         w.u1 = (float)v1;

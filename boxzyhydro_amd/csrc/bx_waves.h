@@ -99,7 +99,7 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, const double *u
         // cons = cons + fluxtmp (flux.f90:205) with cons = source(0)(U^n)
         const double r0 = ui[0], m0 = ui[1], m1 = ui[2], m2 = ui[3];
         const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r0;
-        const double E1 = vmax(ui[4] - ek, a.seps) + ek;
+        const double E1 = floor_at(ui[4] - ek, a.seps) + ek;
         U[0] = r0 + ft[0];
         U[1] = m0 + ft[1];
         U[2] = m1 + ft[2];
@@ -109,13 +109,13 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, const double *u
         // cons = half*(cons_old+cons+fluxtmp) (flux.f90:213), cons_old = source(0)(U^n)
         const double r0 = uo[0], m0 = uo[1], m1 = uo[2], m2 = uo[3];
         const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r0;
-        const double E1 = vmax(uo[4] - ek, a.seps) + ek;
+        const double E1 = floor_at(uo[4] - ek, a.seps) + ek;
         const double old[5] = {r0, m0, m1, m2, E1};
         for (int i = 0; i < 5; i++) U[i] = 0.5 * (old[i] + ui[i] + ft[i]);
     }
     // floors, flux.f90:225-228
     if (!(U[0] > a.srho) || U[4] < 0.0) flg |= FLAG_FLOOR;
-    U[0] = vmax(U[0], a.srho);
+    U[0] = floor_at(U[0], a.srho);
     if (U[4] < 0.0) U[4] = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0] + a.seps;
     if constexpr (STAGE == 2 && sizeof(S) == 8) {
         // the donor of a two-level corner ghost: park the result before source(1)
@@ -135,7 +135,7 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, const double *u
         const double w0 = q0.u0, w1 = q0.u1, w2 = q0.u2;
         const double ekin = r1 * 0.5 * (w0 * w0 + w1 * w1 + w2 * w2);
         if (!(E1 - ekin > a.seps)) flg |= FLAG_FLOOR;
-        elem<S>(a.pout, c, coff) = (S)(vmax(E1 - ekin, a.seps) * a.gm1);
+        elem<S>(a.pout, c, coff) = (S)(floor_at(E1 - ekin, a.seps) * a.gm1);
         for (int i = 0; i < 5; i++) elem<S>(a.uout, (int64_t)i * n + c, coff) = Us[i];
     } else {
         // main.f90:374-515: state, velocity, source(1), state, velocity
@@ -149,7 +149,7 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, const double *u
         if (fabs(v0) > a.vlimit || fabs(v1) > a.vlimit || fabs(v2) > a.vlimit) flg |= FLAG_CLAMP;
         const double ek = quo[3];
         if (!(U[4] - ek > a.seps)) flg |= FLAG_FLOOR;
-        U[4] = vmax(U[4] - ek, a.seps) + ek;
+        U[4] = floor_at(U[4] - ek, a.seps) + ek;
         S Us[5];
         for (int i = 0; i < 5; i++) {
             Us[i] = (S)U[i];
@@ -163,7 +163,7 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, const double *u
             w1 = v1;
             w2 = v2;
             const double ekin = U[0] * 0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
-            p = vmax(U[4] - ekin, a.seps) * a.gm1;
+            p = floor_at(U[4] - ekin, a.seps) * a.gm1;
         } else {
             rr = Us[0];
             const double inv = 1.0 / rr;
@@ -171,9 +171,11 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, const double *u
             w1 = (double)Us[2] * inv;
             w2 = (double)Us[3] * inv;
             const double ekin = 0.5 * ((double)Us[1] * w0 + (double)Us[2] * w1 + (double)Us[3] * w2);
-            p = vmax((double)Us[4] - ekin, a.seps) * a.gm1;
+            p = floor_at((double)Us[4] - ekin, a.seps) * a.gm1;
         }
-        const double snd1 = sqrt(a.gam * p / rr);
+        const double sq = a.gam * p / rr;
+        double snd1;
+        sqrt_n<1>(&sq, &snd1);
         const double rt = div_by(snd1 + fabs(w0), a.ds[0], a.inv_ds[0], a.ds_pow2[0]) +
                           div_by(snd1 + fabs(w1), a.ds[1], a.inv_ds[1], a.ds_pow2[1]) +
                           div_by(snd1 + fabs(w2), a.ds[2], a.inv_ds[2], a.ds_pow2[2]);
