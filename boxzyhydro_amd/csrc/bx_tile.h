@@ -193,7 +193,9 @@ __device__ __forceinline__ void tile_owner(const StageArgs &a, TileLds<S, TY> &L
             const W6<S> q1 = q2;
             cs0 = cs1;
             const Raw<S> cur = nxt;
-            if (k + 1 < ke) nxt = load_raw<STAGE, S>(a, zplane(k + 3), coff);  // plane k+3 <= nz-1
+            // plane k+3 (the last iteration reloads a plane, so that the queue
+            // needs no branch)
+            nxt = load_raw<STAGE, S>(a, zplane(min(k + 3, nz - 1)), coff);
             q2 = prims<STAGE, S>(a, cur, flg);
             cs1 = sound_of(q1, a.gam);
             Side<S> zmi1, zpl1;
@@ -210,7 +212,7 @@ __device__ __forceinline__ void tile_owner(const StageArgs &a, TileLds<S, TY> &L
             cs0 = cs1;
             if (k + 1 < ke) {
                 const Raw<S> cur = nxt;
-                if (k + 2 < ke) nxt = load_raw<STAGE, S>(a, zplane(k + 2), coff);
+                nxt = load_raw<STAGE, S>(a, zplane(min(k + 2, ke - 1)), coff);
                 const W6<S> w = prims<STAGE, S>(a, cur, flg);
                 cs1 = sound_of(w, a.gam);
                 store_w(L.P[t ^ 1], c, w);
@@ -296,7 +298,7 @@ __device__ __forceinline__ void tile_halo(const StageArgs &a, TileLds<S, TY> &L,
         if (k + 1 < ke) {
             // A: primitives of plane k+1 into the other tile
             const Raw<S> cur = nxt;
-            if (k + 2 < ke) nxt = load_raw<STAGE, S>(a, zplane(k + 2), coff);
+            nxt = load_raw<STAGE, S>(a, zplane(min(k + 2, ke - 1)), coff);
             store_w(L.P[t ^ 1], c, prims<STAGE, S>(a, cur, flg));
         }
         Side<S> ymi;
@@ -389,11 +391,11 @@ __device__ __forceinline__ void tile_edge(const StageArgs &a, TileLds<S, TY> &L,
         if (more) {
             // A: primitives of plane k+1, its own row and the x-edges
             const Raw<S> cur = nxt;
-            if (k + 2 < ke) nxt = load_raw<STAGE, S>(a, zplane(k + 2), coff);
+            nxt = load_raw<STAGE, S>(a, zplane(min(k + 2, ke - 1)), coff);
             store_w(L.P[(k + 1) & 1], c, prims<STAGE, S>(a, cur, flg));
             if (e_on) {
                 const Raw<S> ecur = enxt;
-                if (k + 2 < ke) enxt = load_raw<STAGE, S>(a, zplane(k + 2), eoff);
+                enxt = load_raw<STAGE, S>(a, zplane(min(k + 2, ke - 1)), eoff);
                 edge_prims(k + 1, ecur);
                 edge_states(k + 1);  // B
             }
