@@ -221,7 +221,7 @@ int launch_materialize(bx_ctx *c)
     hipLaunchKernelGGL(k_materialize, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cons, c->u, c->p,
                        c->adindx, c->prm.gammafix, c->prm.small_eps);
     BX_HIP(hipGetLastError());
-    if (c->nd2 > 0) {
+    if (c->nd2 > 0 && c->prm.precision == BX_PREC_FP64) {  // the mixed form copies the donors
         hipLaunchKernelGGL(k_corner_pressure, dim3(1), dim3(64), 0, c->stream, c->nd2, c->d2_dst, c->d2_ubar, c->p);
         BX_HIP(hipGetLastError());
     }
@@ -234,8 +234,6 @@ int fused_supported(const bx_ctx *c)
     if (c->nscratch > 0 && !c->fghost_ok)
         return set_error(BX_ERR_UNSUPPORTED, "fused step needs the ghost donors of a pair of set_ghost_cells calls "
                                              "to resolve to interior cells within two levels");
-    if (c->nd2 > 0 && c->prm.precision != BX_PREC_FP64)
-        return set_error(BX_ERR_UNSUPPORTED, "fused step: two-level corner ghosts in the FP64 form only");
     if (c->grav[0] != 0.0 || c->grav[1] != 0.0 || c->grav[2] != 0.0)
         return set_error(BX_ERR_UNSUPPORTED, "fused step assumes zero gravity");
     if (c->prm.fluxangmom) return set_error(BX_ERR_UNSUPPORTED, "fused step fluxes linear momentum (fluxangmom off)");
@@ -244,12 +242,12 @@ int fused_supported(const bx_ctx *c)
     return BX_OK;
 }
 
-// Default work split of the stage kernels.  FP64: one kernel per stage
-// (64 x 8 tiles, variant 11) with z-chunks of 64 planes, shortened until the
-// grid has two workgroups per CU; the mixed form: the two-kernel stage (64
-// rows and 16 planes per wave, y-term loaded early; autotuned on the 256^3
-// box), shortened on smaller grids until the y kernel has about 2 and the
-// x/z kernel about 4 waves per SIMD to run.  Allocates nothing.
+// Default work split of the stage kernels: one kernel per stage, FP64 on
+// 64 x 8 tiles (variant 11) with z-chunks of 64 planes, the mixed form on
+// 64 x 4 tiles (variant 10, its FP32 tiles being small enough for several
+// workgroups per CU) with chunks of 32; the chunks are shortened until the
+// grid has two workgroups per CU.  The y chunk (fused_ly) only serves the
+// two-kernel variants.  Allocates nothing.
 void fused_defaults(bx_ctx *c)
 {
     const int64_t want = 2 * 1024;
@@ -261,11 +259,11 @@ void fused_defaults(bx_ctx *c)
     }
     if (c->fused_lz == 0) {
         if (c->prm.precision == BX_PREC_MIXED) {
-            int lz = 16;
-            const int64_t rowwaves = ((int64_t)(c->ny - 4) * c->nx + kSegOut - 1) / kSegOut;
-            while (lz > 4 && rowwaves * ((nzi + lz - 1) / lz) < 2 * want) lz /= 2;
+            int lz = 32;
+            const int64_t tiles = (int64_t)((c->nx - 4 + 63) / 64) * ((c->ny - 4 + 3) / 4);
+            while (lz > 4 && tiles * ((nzi + lz - 1) / lz) < 512) lz /= 2;
             c->fused_lz = lz;
-            c->fused_occ = 1;
+            c->fused_occ = 10;
         } else {
             int lz = 64;
             const int64_t tiles = (int64_t)((c->nx - 4 + 63) / 64) * ((c->ny - 4 + 7) / 8);
