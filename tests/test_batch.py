@@ -238,3 +238,60 @@ def test_batch_result_independent_of_batch_and_order():
         r = b2.result()[j]
         for k in ("cons", "u", "p"):
             assert _same(r[k], full[i][k])
+
+
+def _long_tubes():
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tubes_long.json")) as f:
+        return json.load(f)
+
+
+def test_long_tube_goldens_match_generator():
+    """The three long-tube goldens (tests/golden/make_tube_golden.py: the
+    tubes that set the 512-tube batch's run time, reference runs of their full
+    grids) were made from this generator's set-ups."""
+    from boxzyhydro_amd.batch import grid_of_row
+    from cases import ic_digest
+
+    recs = _long_tubes()
+    allt = ics.riemann_tubes(512, seed=0)
+    assert sorted(r["generator"]["index"] for r in recs.values()) == [26, 156, 241]
+    for rec in recs.values():
+        prm, row = allt[rec["generator"]["index"]]
+        assert ic_digest(grid_of_row(prm, row)) == rec["ic_sha256"]
+        assert len(rec["dt_sampled"]) == (rec["steps"] + rec["dt_every"] - 1) // rec["dt_every"]
+
+
+@pytest.mark.gpu
+def test_batch_long_tubes_match_reference_goldens():
+    """Inside the whole 512-tube batch, the three tubes that set its run time
+    (up to 4,049 cells and 16,891 steps) against the reference's runs of their
+    full grids: step count, every dt (SHA-256 of the sequence, and every 64th
+    value), final time, and the final cons/u/p of the expanded rows."""
+    import hashlib
+
+    from boxzyhydro_amd.batch import expand_row
+
+    recs = _long_tubes()
+    allt = ics.riemann_tubes(512, seed=0)
+    b = _batch(allt)
+    b.log_dt(max(r["steps"] for r in recs.values()))
+    print("512 tubes, long-tube goldens", flush=True)
+    b.run(graph=True)
+    b.sync()
+    res = b.result()
+    dts = b.dts()
+    for name, rec in recs.items():
+        i = rec["generator"]["index"]
+        prm, _ = allt[i]
+        r, d = res[i], np.asarray(dts[i], np.float64)
+        print(f"  {name}: {r['steps']} steps", flush=True)
+        assert r["steps"] == rec["steps"], name
+        assert list(d[::rec["dt_every"]]) == rec["dt_sampled"], name
+        assert hashlib.sha256(d.tobytes()).hexdigest() == rec["dt_sha256"], name
+        assert r["time"] == rec["time_final"], name
+        full = expand_row(prm, r)
+        for k in ("cons", "u", "p"):
+            assert sha(full[k]) == rec["final_sha256"][k], (name, k)
