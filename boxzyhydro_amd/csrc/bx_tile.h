@@ -109,20 +109,40 @@ __device__ __forceinline__ Side<S> fetch_side(const S (&B)[6][N], int idx, S gam
     return s;
 }
 
+// 32-bit whole-wave shift by one lane (DPP wave_shr:1 / wave_shl:1): the lane
+// with no source lane (0 / 63) keeps `edge`, the others take their lower /
+// higher neighbour's `v`.  No LDS round trip, unlike a lane permute.
+template <int CTRL>
+__device__ __forceinline__ int wave_shift(int v, int edge)
+{
+    return __builtin_amdgcn_update_dpp(edge, v, CTRL, 0xf, 0xf, false);
+}
+
+template <int CTRL, typename T>
+__device__ __forceinline__ T wave_shift_t(T v, T edge)
+{
+    if constexpr (sizeof(T) == 8) {
+        const long long vb = __double_as_longlong(v), eb = __double_as_longlong(edge);
+        const int lo = wave_shift<CTRL>((int)vb, (int)eb);
+        const int hi = wave_shift<CTRL>((int)(vb >> 32), (int)(eb >> 32));
+        return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+    } else {
+        return __int_as_float(wave_shift<CTRL>(__float_as_int(v), __float_as_int(edge)));
+    }
+}
+
 // value of the next lower lane; lane 0 takes `edge`
 template <typename T>
-__device__ __forceinline__ T from_lower(T v, T edge, int lane)
+__device__ __forceinline__ T from_lower(T v, T edge, int)
 {
-    const T s = __shfl(v, (lane + 63) & 63, 64);
-    return lane == 0 ? edge : s;
+    return wave_shift_t<0x138>(v, edge);  // wave_shr:1
 }
 
 // value of the next higher lane; lane 63 takes `edge`
 template <typename T>
-__device__ __forceinline__ T from_upper(T v, T edge, int lane)
+__device__ __forceinline__ T from_upper(T v, T edge, int)
 {
-    const T s = __shfl(v, (lane + 1) & 63, 64);
-    return lane == 63 ? edge : s;
+    return wave_shift_t<0x130>(v, edge);  // wave_shl:1
 }
 
 // Per-tile constants of one stage.
