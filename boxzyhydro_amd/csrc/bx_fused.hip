@@ -34,12 +34,18 @@
 namespace bx {
 
 // One stage as one kernel: x-y tile of 64 x TY outputs with its +-2 halo in
-// LDS, marching in z (bx_tile.h: tile_stage)
-template <int STAGE, typename S, int TY>
+// LDS, marching in z (bx_tile.h: tile_stage).  P2: every spacing and the
+// cell volume are powers of two (their divisions are multiplications), known
+// at compile time, which keeps the spacings themselves out of the registers.
+template <int STAGE, typename S, int TY, bool P2>
 __global__ __launch_bounds__(64 * (TY + 4)) void k_stage(StageArgs a0)
 {
     __shared__ TileLds<S, TY> lds;
-    const StageArgs a = with_dt(a0);
+    StageArgs a = with_dt(a0);
+    if constexpr (P2) {
+        a.ds_pow2[0] = a.ds_pow2[1] = a.ds_pow2[2] = 1;
+        a.vol_pow2 = 1;
+    }
     // Workgroups go to the 8 XCDs in turn (workgroup id mod 8).  Give each
     // XCD a contiguous range of tiles, x fastest, so that the tiles in flight
     // on one XCD are neighbours and share their halo rows and boundary cache
@@ -161,11 +167,15 @@ StageArgs base_args(bx_ctx *c, double dt)
 template <int STAGE, typename S, int TY>
 static int launch_tile(bx_ctx *c, const StageArgs &a)
 {
+    const bool p2 = a.ds_pow2[0] && a.ds_pow2[1] && a.ds_pow2[2] && a.vol_pow2;
     const int nzi = a.zhi - a.zlo;
     const unsigned gx = (unsigned)((c->nx - 4 + 63) / 64);
     const unsigned gy = (unsigned)((c->ny - 4 + TY - 1) / TY);
     const unsigned gz = (unsigned)((nzi + a.lz - 1) / a.lz);
-    hipLaunchKernelGGL((k_stage<STAGE, S, TY>), dim3(gx, gy, gz), dim3(64 * (TY + 4)), 0, c->stream, a);
+    if (p2)
+        hipLaunchKernelGGL((k_stage<STAGE, S, TY, true>), dim3(gx, gy, gz), dim3(64 * (TY + 4)), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_stage<STAGE, S, TY, false>), dim3(gx, gy, gz), dim3(64 * (TY + 4)), 0, c->stream, a);
     BX_HIP(hipGetLastError());
     return BX_OK;
 }

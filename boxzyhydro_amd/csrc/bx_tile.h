@@ -364,7 +364,6 @@ __device__ __forceinline__ void tile_edge(const StageArgs &a, TileLds<S, TY> &L,
     const int ex = g.x0 + ecol, ey = g.j0 + erow;
     const unsigned eoff = (unsigned)(min(ey, ny - 1) * nx + min(max(ex, 0), nx - 1)) * (unsigned)sizeof(S);
     const bool e_xy_ghost = ex < 2 || ex > nx - 3 || ey < 2 || ey > ny - 3;
-    const int lm1 = (lane + 63) & 63, lp1 = (lane + 1) & 63;
 
     // the three parts of the x-edge items of plane k
     W6<S> ew{};
@@ -373,19 +372,41 @@ __device__ __forceinline__ void tile_edge(const StageArgs &a, TileLds<S, TY> &L,
         ew = prims<STAGE, S>(a, raw, flg);
         if (ej == 1 || ej == 5) store_w(L.P[k & 1], (erow + 2) * RS + ecol + 1, ew);
     };
+    // neighbours within a row's seven lanes through whole-wave shifts (the
+    // lanes whose neighbour lies in another row or outside the wave are not
+    // used: columns -1, 63 and 64 take columns -2..0 and 62..65)
+    auto lower_w = [](const W6<S> &w) {
+        W6<S> o;
+        o.d = wave_shift_t<0x138>(w.d, w.d);
+        o.u0 = wave_shift_t<0x138>(w.u0, w.u0);
+        o.u1 = wave_shift_t<0x138>(w.u1, w.u1);
+        o.u2 = wave_shift_t<0x138>(w.u2, w.u2);
+        o.p = wave_shift_t<0x138>(w.p, w.p);
+        o.e = wave_shift_t<0x138>(w.e, w.e);
+        return o;
+    };
+    auto upper_w = [](const W6<S> &w) {
+        W6<S> o;
+        o.d = wave_shift_t<0x130>(w.d, w.d);
+        o.u0 = wave_shift_t<0x130>(w.u0, w.u0);
+        o.u1 = wave_shift_t<0x130>(w.u1, w.u1);
+        o.u2 = wave_shift_t<0x130>(w.u2, w.u2);
+        o.p = wave_shift_t<0x130>(w.p, w.p);
+        o.e = wave_shift_t<0x130>(w.e, w.e);
+        return o;
+    };
     auto edge_states = [&](int k) {
-        dir_states<0>(shfl_w(ew, lm1), ew, shfl_w(ew, lp1), sound_of(ew, a.gam), e_xy_ghost || zghost(k), a, emi,
-                      epl);
+        dir_states<0>(lower_w(ew), ew, upper_w(ew), sound_of(ew, a.gam), e_xy_ghost || zghost(k), a, emi, epl);
     };
     auto edge_publish = [&]() {
         if (ej == 1) store_side(L.SxL, erow, epl);
         Side<S> xl;
-        xl.d = shfl_v(epl.d, lm1);
-        xl.v[0] = shfl_v(epl.v[0], lm1);
-        xl.v[1] = shfl_v(epl.v[1], lm1);
-        xl.v[2] = shfl_v(epl.v[2], lm1);
-        xl.p = shfl_v(epl.p, lm1);
-        xl.e = shfl_v(epl.e, lm1);
+        xl.d = wave_shift_t<0x138>(epl.d, epl.d);
+        xl.v[0] = wave_shift_t<0x138>(epl.v[0], epl.v[0]);
+        xl.v[1] = wave_shift_t<0x138>(epl.v[1], epl.v[1]);
+        xl.v[2] = wave_shift_t<0x138>(epl.v[2], epl.v[2]);
+        xl.p = wave_shift_t<0x138>(epl.p, epl.p);
+        xl.e = wave_shift_t<0x138>(epl.e, epl.e);
         xl.g = gamS;
         if (ej == 5) {
             S F[5];
