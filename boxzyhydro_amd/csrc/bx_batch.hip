@@ -108,7 +108,7 @@ __device__ __forceinline__ double eos_state(const Tube &t, double rho, double &E
                                             double pold, unsigned &flg)
 {
     const double ekin = rho * 0.5 * (u0 * u0 + u1 * u1 + u2 * u2);
-    double eps = vmax(E - ekin, t.seps);
+    double eps = floor_at(E - ekin, t.seps);
     if (rho < 1e-4) {
         flg |= FLAG_LOWRHO;
         const double tk = vmin(5e3, pold / (rho * t.rgas) * t.muc);
@@ -139,7 +139,7 @@ __device__ __forceinline__ void velocity_of(const Tube &t, double U[5], double v
         }
         const double E = U[4] - ekin;
         ekin = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / rho;
-        U[4] = vmax(E, t.seps) + ekin;
+        U[4] = floor_at(E, t.seps) + ekin;
     }
 }
 
@@ -147,13 +147,15 @@ __device__ __forceinline__ void velocity_of(const Tube &t, double U[5], double v
 __device__ __forceinline__ double source_E(const Tube &t, const double U[5])
 {
     const double ek = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0];
-    return vmax(U[4] - ek, t.seps) + ek;
+    return floor_at(U[4] - ek, t.seps) + ek;
 }
 
 // courant.f90:20-21 for one cell
 __device__ __forceinline__ double rate_of(const Tube &t, double rho, double p, double u0, double u1, double u2)
 {
-    const double sound = sqrt(t.gam * p / rho);
+    const double sq = t.gam * p / rho;
+    double sound;
+    sqrt_n<1>(&sq, &sound);
     const double r0 = (sound + fabs(u0)) / t.dx;
     const double r1 = (sound + fabs(u1)) / t.dy;
     // the same operands give the same quotient: the z term of a cube cell with |u1|==|u2|
@@ -180,23 +182,47 @@ __device__ __forceinline__ double block_max(double v, double *red, int nw)
     return m;
 }
 
+// 64-bit whole-wave shift by one lane (DPP wave_shr:1 = 0x138, from the
+// next lower lane; wave_shl:1 = 0x130, from the next higher lane); the lane
+// with no source keeps its own value.  Only lanes 1..62 of a segment use
+// their neighbours' values (lanes 0, 1, 62 and 63 are its halo).
+template <int CTRL>
+__device__ __forceinline__ double wshift(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+template <int CTRL>
+__device__ __forceinline__ W6<double> wshift_w(const W6<double> &w)
+{
+    W6<double> o;
+    o.d = wshift<CTRL>(w.d);
+    o.u0 = wshift<CTRL>(w.u0);
+    o.u1 = wshift<CTRL>(w.u1);
+    o.u2 = wshift<CTRL>(w.u2);
+    o.p = wshift<CTRL>(w.p);
+    o.e = wshift<CTRL>(w.e);
+    return o;
+}
+
 // x interface states of every lane's cell and the flux of its face x-1/2
 // (the face x+1/2 is the next lane's)
 __device__ __forceinline__ void x_faces(const StageArgs &sa, const W6<double> &q, double snd, bool ghost,
                                         double Fx[5])
 {
-    const int lane = threadIdx.x & 63;
-    const int lm1 = (lane + 63) & 63, lp1 = (lane + 1) & 63;
-    const W6<double> wm = shfl_w(q, lm1), wp = shfl_w(q, lp1);
+    const W6<double> wm = wshift_w<0x138>(q), wp = wshift_w<0x130>(q);
     Side<double> mi, pl;
     dir_states<0>(wm, q, wp, snd, ghost, sa, mi, pl);
     Side<double> L;
-    L.d = shfl_v(pl.d, lm1);
-    L.v[0] = shfl_v(pl.v[0], lm1);
-    L.v[1] = shfl_v(pl.v[1], lm1);
-    L.v[2] = shfl_v(pl.v[2], lm1);
-    L.p = shfl_v(pl.p, lm1);
-    L.e = shfl_v(pl.e, lm1);
+    L.d = wshift<0x138>(pl.d);
+    L.v[0] = wshift<0x138>(pl.v[0]);
+    L.v[1] = wshift<0x138>(pl.v[1]);
+    L.v[2] = wshift<0x138>(pl.v[2]);
+    L.p = wshift<0x138>(pl.p);
+    L.e = wshift<0x138>(pl.e);
     L.g = sa.gam;
     face_flux<double, 0, true>(L, mi, Fx);
 }
@@ -294,11 +320,11 @@ __global__ __launch_bounds__(64 * W) void k_tubes(BatchArgs a, const Tube *__res
                 q.u1 = v[1];
                 q.u2 = v[2];
                 q.p = p314;
-                q.e = vmax(U[4] - 0.5 * U[0] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), t.seps);
+                q.e = floor_at(U[4] - 0.5 * U[0] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), t.seps);
                 double Fx[5];
                 x_faces(sa, q, sound_of(q, t.gam), ghost, Fx);
                 double dF[5];
-                for (int k = 0; k < 5; k++) dF[k] = shfl_v(Fx[k], (lane + 1) & 63) - Fx[k];
+                for (int k = 0; k < 5; k++) dF[k] = wshift<0x130>(Fx[k]) - Fx[k];
                 __asm__ volatile("" ::: "memory");  // keep the update's loads after the faces (register pressure)
                 if (lane >= 2 && lane < 62 && !ghost) {
                     double Us[5], us[3];
@@ -306,7 +332,7 @@ __global__ __launch_bounds__(64 * W) void k_tubes(BatchArgs a, const Tube *__res
                         PL(oldr + k * T, c) = U[k];                     // cons_old, main.f90:327-333
                         Us[k] = U[k] + flux_tmp(t, dF[k], dt);      // flux.f90:205
                     }
-                    Us[0] = vmax(Us[0], t.srho);                    // flux.f90:225-228
+                    Us[0] = floor_at(Us[0], t.srho);                // flux.f90:225-228
                     if (Us[4] < 0.0) Us[4] = 0.5 * (Us[1] * Us[1] + Us[2] * Us[2] + Us[3] * Us[3]) / Us[0] + t.seps;
                     const double ps = eos_state(t, Us[0], Us[4], v[0], v[1], v[2], p314, flg);  // :353
                     velocity_of(t, Us, us, flg);                                             // :355
@@ -330,17 +356,17 @@ __global__ __launch_bounds__(64 * W) void k_tubes(BatchArgs a, const Tube *__res
                 q.u1 = PL(ssr + 6 * T, d);
                 q.u2 = PL(ssr + 7 * T, d);
                 q.p = PL(ssr + 8 * T, d);
-                q.e = vmax(E - 0.5 * q.d * (q.u0 * q.u0 + q.u1 * q.u1 + q.u2 * q.u2), t.seps);
+                q.e = floor_at(E - 0.5 * q.d * (q.u0 * q.u0 + q.u1 * q.u1 + q.u2 * q.u2), t.seps);
                 double Fx[5];
                 x_faces(sa, q, sound_of(q, t.gam), ghost, Fx);
                 double dF[5];
-                for (int k = 0; k < 5; k++) dF[k] = shfl_v(Fx[k], (lane + 1) & 63) - Fx[k];
+                for (int k = 0; k < 5; k++) dF[k] = wshift<0x130>(Fx[k]) - Fx[k];
                 __asm__ volatile("" ::: "memory");  // keep the update's loads after the faces (register pressure)
                 if (lane >= 2 && lane < 62 && !ghost) {
                     double U[5], v[3], v2[3];
                     for (int k = 0; k < 5; k++)
                         U[k] = 0.5 * (PL(oldr + k * T, c) + PL(ssr + k * T, c) + flux_tmp(t, dF[k], dt));  // flux.f90:213
-                    U[0] = vmax(U[0], t.srho);
+                    U[0] = floor_at(U[0], t.srho);
                     if (U[4] < 0.0) U[4] = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0] + t.seps;
                     const double p376 = eos_state(t, U[0], U[4], q.u0, q.u1, q.u2, q.p, flg);    // :376
                     velocity_of(t, U, v, flg);                                                  // :378
