@@ -181,8 +181,10 @@ __device__ __forceinline__ void face_vjp(const Side<double> &L, const Side<doubl
                                          Side<double> &lL, Side<double> &lR)
 {
     const double vl = L.v[dir], vr = R.v[dir];
-    const double cr = sqrt((R.g * R.p) / R.d);
-    const double cl = sqrt((L.g * L.p) / L.d);
+    const double sq[2] = {(R.g * R.p) / R.d, (L.g * L.p) / L.d};
+    double rt[2];
+    sqrt_n<2>(sq, rt);
+    const double cr = rt[0], cl = rt[1];
     const double a1 = vr + cr, a2 = vl + cl;
     const double t1 = vmax(0.0, a1);
     const double ap = vmax(t1, a2);
@@ -319,9 +321,13 @@ __global__ __launch_bounds__(256) void k_adj_march(StageArgs a, const double *__
     wzero(acc0);
     wzero(acc1);
     wzero(acc2);
+    // the raw values of cell s+2 are loaded one step ahead
+    auto R = [&](int j) -> Raw<double> { return load_raw<STAGE, double>(a, U(min(max(j, 0), nd - 1)), xoff); };
+    Raw<double> nraw = R(s0 + 2);
     for (int s = s0; s <= s1; s++) {
         // queue: w0..w2 = W(s-1..s+1), c1 = sound(s); states of s, adjoint of its minus state
-        const W6<double> w3 = W(s + 2);
+        const W6<double> w3 = prims<STAGE, double>(a, nraw, flg);
+        nraw = R(s + 3);
         const double c2 = sound_of(w2, a.gam);
         Side<double> lpl_s, lmi_n;
         lpl_s.d = lpl_s.v[0] = lpl_s.v[1] = lpl_s.v[2] = lpl_s.p = lpl_s.e = lpl_s.g = 0.0;
@@ -334,10 +340,21 @@ __global__ __launch_bounds__(256) void k_adj_march(StageArgs a, const double *__
             for (int k = 0; k < 5; k++) lF[k] = kappa * (gn[k] - gs[k]);
             face_vjp<d>(pl_s, mi_n, lF, lpl_s, lmi_n);
         }
+        // the z march adds cell s-1's adjoint to the y march's: its primitive
+        // adjoints are loaded before the state reverse, so that the load runs
+        // under it
+        const bool emit = xin && s - 1 >= jb && s - 1 < je;
+        const bool assign = d == 1 || ghost(s - 1);
+        double prev[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (emit && !assign)
+            for (int k = 0; k < 6; k++) prev[k] = elem<double>(LW, (int64_t)k * n + U(s - 1), xoff);
         double lsnd = 0.0;
         dir_states_vjp<d>(w0, w1, w2, c1, ghost(s), a, lmi_s, lpl_s, acc0, acc1, acc2, lsnd);
         sound_vjp(w1, c1, a.gam, lsnd, acc1);
-        if (xin && s - 1 >= jb && s - 1 < je) add_to(LW, n, U(s - 1), xoff, acc0, d == 1 || ghost(s - 1));
+        if (emit) {
+            const double v[6] = {acc0.d, acc0.u0, acc0.u1, acc0.u2, acc0.p, acc0.e};
+            for (int k = 0; k < 6; k++) elem<double>(LW, (int64_t)k * n + U(s - 1), xoff) = assign ? v[k] : prev[k] + v[k];
+        }
         // shift the queue
         w0 = w1;
         w1 = w2;
@@ -380,36 +397,42 @@ __global__ __launch_bounds__(256) void k_adj_x(StageArgs a, const double *__rest
     const bool ghost = zg || x < 2 || x > nx - 3 || y < 2 || y > a.ny - 3;
     const unsigned coff = (unsigned)F * 8u;
     const int64_t zb = (int64_t)k * plane;
-    const int lm1 = (lane + 63) & 63, lp1 = (lane + 1) & 63;
     unsigned int flg = 0;
     const W6<double> q = load_prims<STAGE, double>(a, zb, coff, flg);
     const double snd = sound_of(q, a.gam);
     double g[5];
     for (int i = 0; i < 5; i++) g[i] = gscale * elem<double>(lam, (int64_t)i * n + zb, coff);
-    const W6<double> wm = shfl_w(q, lm1), wp = shfl_w(q, lp1);
+    const W6<double> wm = lane_below_w(q), wp = lane_above_w(q);
     Side<double> mi, pl;
     dir_states<0>(wm, q, wp, snd, ghost, a, mi, pl);
     // face x-1/2 of this lane: L = plus state of lane-1, R = own minus state
     Side<double> L;
-    L.d = shfl_v(pl.d, lm1);
-    L.v[0] = shfl_v(pl.v[0], lm1);
-    L.v[1] = shfl_v(pl.v[1], lm1);
-    L.v[2] = shfl_v(pl.v[2], lm1);
-    L.p = shfl_v(pl.p, lm1);
-    L.e = shfl_v(pl.e, lm1);
+    L.d = lane_below(pl.d);
+    L.v[0] = lane_below(pl.v[0]);
+    L.v[1] = lane_below(pl.v[1]);
+    L.v[2] = lane_below(pl.v[2]);
+    L.p = lane_below(pl.p);
+    L.e = lane_below(pl.e);
     L.g = a.gam;
     double lF[5];
-    for (int i = 0; i < 5; i++) lF[i] = kappa * (g[i] - shfl_v(g[i], lm1));
+    for (int i = 0; i < 5; i++) lF[i] = kappa * (g[i] - lane_below(g[i]));
     Side<double> lL, lmi;
     face_vjp<0>(L, mi, lF, lL, lmi);
     Side<double> lpl;  // the face x+1/2 belongs to lane+1
-    lpl.d = shfl_v(lL.d, lp1);
-    lpl.v[0] = shfl_v(lL.v[0], lp1);
-    lpl.v[1] = shfl_v(lL.v[1], lp1);
-    lpl.v[2] = shfl_v(lL.v[2], lp1);
-    lpl.p = shfl_v(lL.p, lp1);
-    lpl.e = shfl_v(lL.e, lp1);
+    lpl.d = lane_above(lL.d);
+    lpl.v[0] = lane_above(lL.v[0]);
+    lpl.v[1] = lane_above(lL.v[1]);
+    lpl.v[2] = lane_above(lL.v[2]);
+    lpl.p = lane_above(lL.p);
+    lpl.e = lane_above(lL.e);
     lpl.g = 0.0;
+    // the y and z marches' primitive adjoints of this cell, loaded before the
+    // state reverse so that the load runs under it (x-ghost cells assign)
+    const bool emit = lane >= 3 && lane <= 60 && F0 + lane < (int64_t)(a.ny - 2) * nx;
+    const bool assign = x < 2 || x > nx - 3;
+    double prev[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (emit && !assign)
+        for (int k = 0; k < 6; k++) prev[k] = elem<double>(LW, (int64_t)k * n + zb, coff);
     W6<double> gm, gc, gp;
     wzero(gm);
     wzero(gc);
@@ -418,15 +441,17 @@ __global__ __launch_bounds__(256) void k_adj_x(StageArgs a, const double *__rest
     dir_states_vjp<0>(wm, q, wp, snd, ghost, a, lmi, lpl, gm, gc, gp, lsnd);
     sound_vjp(q, snd, a.gam, lsnd, gc);
     // own adjoint + the wp part of lane-1 + the wm part of lane+1
-    const W6<double> fromL = shfl_w(gp, lm1), fromR = shfl_w(gm, lp1);
+    const W6<double> fromL = lane_below_w(gp), fromR = lane_above_w(gm);
     gc.d += fromL.d + fromR.d;
     gc.u0 += fromL.u0 + fromR.u0;
     gc.u1 += fromL.u1 + fromR.u1;
     gc.u2 += fromL.u2 + fromR.u2;
     gc.p += fromL.p + fromR.p;
     gc.e += fromL.e + fromR.e;
-    const bool emit = lane >= 3 && lane <= 60 && F0 + lane < (int64_t)(a.ny - 2) * nx;
-    if (emit) add_to(LW, n, zb, coff, gc, x < 2 || x > nx - 3);
+    if (emit) {
+        const double v[6] = {gc.d, gc.u0, gc.u1, gc.u2, gc.p, gc.e};
+        for (int k = 0; k < 6; k++) elem<double>(LW, (int64_t)k * n + zb, coff) = assign ? v[k] : prev[k] + v[k];
+    }
 }
 
 // ------------------------------------------------------- pointwise kernels

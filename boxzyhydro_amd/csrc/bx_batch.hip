@@ -182,47 +182,23 @@ __device__ __forceinline__ double block_max(double v, double *red, int nw)
     return m;
 }
 
-// 64-bit whole-wave shift by one lane (DPP wave_shr:1 = 0x138, from the
-// next lower lane; wave_shl:1 = 0x130, from the next higher lane); the lane
-// with no source keeps its own value.  Only lanes 1..62 of a segment use
-// their neighbours' values (lanes 0, 1, 62 and 63 are its halo).
-template <int CTRL>
-__device__ __forceinline__ double wshift(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xf, 0xf, false);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
-template <int CTRL>
-__device__ __forceinline__ W6<double> wshift_w(const W6<double> &w)
-{
-    W6<double> o;
-    o.d = wshift<CTRL>(w.d);
-    o.u0 = wshift<CTRL>(w.u0);
-    o.u1 = wshift<CTRL>(w.u1);
-    o.u2 = wshift<CTRL>(w.u2);
-    o.p = wshift<CTRL>(w.p);
-    o.e = wshift<CTRL>(w.e);
-    return o;
-}
-
+// Only lanes 1..62 of a segment use their neighbours' values (lanes 0, 1, 62
+// and 63 are its halo), so whole-wave shifts serve for the x-neighbours.
 // x interface states of every lane's cell and the flux of its face x-1/2
 // (the face x+1/2 is the next lane's)
 __device__ __forceinline__ void x_faces(const StageArgs &sa, const W6<double> &q, double snd, bool ghost,
                                         double Fx[5])
 {
-    const W6<double> wm = wshift_w<0x138>(q), wp = wshift_w<0x130>(q);
+    const W6<double> wm = lane_below_w(q), wp = lane_above_w(q);
     Side<double> mi, pl;
     dir_states<0>(wm, q, wp, snd, ghost, sa, mi, pl);
     Side<double> L;
-    L.d = wshift<0x138>(pl.d);
-    L.v[0] = wshift<0x138>(pl.v[0]);
-    L.v[1] = wshift<0x138>(pl.v[1]);
-    L.v[2] = wshift<0x138>(pl.v[2]);
-    L.p = wshift<0x138>(pl.p);
-    L.e = wshift<0x138>(pl.e);
+    L.d = lane_below(pl.d);
+    L.v[0] = lane_below(pl.v[0]);
+    L.v[1] = lane_below(pl.v[1]);
+    L.v[2] = lane_below(pl.v[2]);
+    L.p = lane_below(pl.p);
+    L.e = lane_below(pl.e);
     L.g = sa.gam;
     face_flux<double, 0, true>(L, mi, Fx);
 }
@@ -324,7 +300,7 @@ __global__ __launch_bounds__(64 * W) void k_tubes(BatchArgs a, const Tube *__res
                 double Fx[5];
                 x_faces(sa, q, sound_of(q, t.gam), ghost, Fx);
                 double dF[5];
-                for (int k = 0; k < 5; k++) dF[k] = wshift<0x130>(Fx[k]) - Fx[k];
+                for (int k = 0; k < 5; k++) dF[k] = lane_above(Fx[k]) - Fx[k];
                 __asm__ volatile("" ::: "memory");  // keep the update's loads after the faces (register pressure)
                 if (lane >= 2 && lane < 62 && !ghost) {
                     double Us[5], us[3];
@@ -360,7 +336,7 @@ __global__ __launch_bounds__(64 * W) void k_tubes(BatchArgs a, const Tube *__res
                 double Fx[5];
                 x_faces(sa, q, sound_of(q, t.gam), ghost, Fx);
                 double dF[5];
-                for (int k = 0; k < 5; k++) dF[k] = wshift<0x130>(Fx[k]) - Fx[k];
+                for (int k = 0; k < 5; k++) dF[k] = lane_above(Fx[k]) - Fx[k];
                 __asm__ volatile("" ::: "memory");  // keep the update's loads after the faces (register pressure)
                 if (lane >= 2 && lane < 62 && !ghost) {
                     double U[5], v[3], v2[3];

@@ -76,6 +76,35 @@ __device__ __forceinline__ T div_by(T x, double d, double inv, int pow2)
     return pow2 ? x * T(inv) : Arith<T>::div(x, T(d));
 }
 
+// 32-bit whole-wave shift by one lane (DPP wave_shr:1 / wave_shl:1): the lane
+// with no source lane (0 / 63) keeps `edge`, the others take their lower /
+// higher neighbour's `v`.  No LDS round trip, unlike a lane permute.
+template <int CTRL>
+__device__ __forceinline__ int wave_shift(int v, int edge)
+{
+    return __builtin_amdgcn_update_dpp(edge, v, CTRL, 0xf, 0xf, false);
+}
+
+template <int CTRL, typename T>
+__device__ __forceinline__ T wave_shift_t(T v, T edge)
+{
+    if constexpr (sizeof(T) == 8) {
+        const long long vb = __double_as_longlong(v), eb = __double_as_longlong(edge);
+        const int lo = wave_shift<CTRL>((int)vb, (int)eb);
+        const int hi = wave_shift<CTRL>((int)(vb >> 32), (int)(eb >> 32));
+        return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+    } else {
+        return __int_as_float(wave_shift<CTRL>(__float_as_int(v), __float_as_int(edge)));
+    }
+}
+
+// the value of the next lower / higher lane (a lane with no such lane keeps
+// its own)
+template <typename T>
+__device__ __forceinline__ T lane_below(T v) { return wave_shift_t<0x138>(v, v); }
+template <typename T>
+__device__ __forceinline__ T lane_above(T v) { return wave_shift_t<0x130>(v, v); }
+
 template <typename T>
 __device__ __forceinline__ T shfl_v(T v, int src)
 {
@@ -94,6 +123,23 @@ __device__ __forceinline__ W6<T> shfl_w(const W6<T> &w, int src)
     o.e = shfl_v(w.e, src);
     return o;
 }
+
+template <int CTRL, typename T>
+__device__ __forceinline__ W6<T> wave_shift_w(const W6<T> &w)
+{
+    W6<T> o;
+    o.d = wave_shift_t<CTRL>(w.d, w.d);
+    o.u0 = wave_shift_t<CTRL>(w.u0, w.u0);
+    o.u1 = wave_shift_t<CTRL>(w.u1, w.u1);
+    o.u2 = wave_shift_t<CTRL>(w.u2, w.u2);
+    o.p = wave_shift_t<CTRL>(w.p, w.p);
+    o.e = wave_shift_t<CTRL>(w.e, w.e);
+    return o;
+}
+template <typename T>
+__device__ __forceinline__ W6<T> lane_below_w(const W6<T> &w) { return wave_shift_w<0x138>(w); }
+template <typename T>
+__device__ __forceinline__ W6<T> lane_above_w(const W6<T> &w) { return wave_shift_w<0x130>(w); }
 
 template <typename T>
 __device__ __forceinline__ T comp(const W6<T> &w, int j)

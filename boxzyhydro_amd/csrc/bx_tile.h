@@ -109,28 +109,6 @@ __device__ __forceinline__ Side<S> fetch_side(const S (&B)[6][N], int idx, S gam
     return s;
 }
 
-// 32-bit whole-wave shift by one lane (DPP wave_shr:1 / wave_shl:1): the lane
-// with no source lane (0 / 63) keeps `edge`, the others take their lower /
-// higher neighbour's `v`.  No LDS round trip, unlike a lane permute.
-template <int CTRL>
-__device__ __forceinline__ int wave_shift(int v, int edge)
-{
-    return __builtin_amdgcn_update_dpp(edge, v, CTRL, 0xf, 0xf, false);
-}
-
-template <int CTRL, typename T>
-__device__ __forceinline__ T wave_shift_t(T v, T edge)
-{
-    if constexpr (sizeof(T) == 8) {
-        const long long vb = __double_as_longlong(v), eb = __double_as_longlong(edge);
-        const int lo = wave_shift<CTRL>((int)vb, (int)eb);
-        const int hi = wave_shift<CTRL>((int)(vb >> 32), (int)(eb >> 32));
-        return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-    } else {
-        return __int_as_float(wave_shift<CTRL>(__float_as_int(v), __float_as_int(edge)));
-    }
-}
-
 // value of the next lower lane; lane 0 takes `edge`
 template <typename T>
 __device__ __forceinline__ T from_lower(T v, T edge, int)
@@ -375,38 +353,18 @@ __device__ __forceinline__ void tile_edge(const StageArgs &a, TileLds<S, TY> &L,
     // neighbours within a row's seven lanes through whole-wave shifts (the
     // lanes whose neighbour lies in another row or outside the wave are not
     // used: columns -1, 63 and 64 take columns -2..0 and 62..65)
-    auto lower_w = [](const W6<S> &w) {
-        W6<S> o;
-        o.d = wave_shift_t<0x138>(w.d, w.d);
-        o.u0 = wave_shift_t<0x138>(w.u0, w.u0);
-        o.u1 = wave_shift_t<0x138>(w.u1, w.u1);
-        o.u2 = wave_shift_t<0x138>(w.u2, w.u2);
-        o.p = wave_shift_t<0x138>(w.p, w.p);
-        o.e = wave_shift_t<0x138>(w.e, w.e);
-        return o;
-    };
-    auto upper_w = [](const W6<S> &w) {
-        W6<S> o;
-        o.d = wave_shift_t<0x130>(w.d, w.d);
-        o.u0 = wave_shift_t<0x130>(w.u0, w.u0);
-        o.u1 = wave_shift_t<0x130>(w.u1, w.u1);
-        o.u2 = wave_shift_t<0x130>(w.u2, w.u2);
-        o.p = wave_shift_t<0x130>(w.p, w.p);
-        o.e = wave_shift_t<0x130>(w.e, w.e);
-        return o;
-    };
     auto edge_states = [&](int k) {
-        dir_states<0>(lower_w(ew), ew, upper_w(ew), sound_of(ew, a.gam), e_xy_ghost || zghost(k), a, emi, epl);
+        dir_states<0>(lane_below_w(ew), ew, lane_above_w(ew), sound_of(ew, a.gam), e_xy_ghost || zghost(k), a, emi, epl);
     };
     auto edge_publish = [&]() {
         if (ej == 1) store_side(L.SxL, erow, epl);
         Side<S> xl;
-        xl.d = wave_shift_t<0x138>(epl.d, epl.d);
-        xl.v[0] = wave_shift_t<0x138>(epl.v[0], epl.v[0]);
-        xl.v[1] = wave_shift_t<0x138>(epl.v[1], epl.v[1]);
-        xl.v[2] = wave_shift_t<0x138>(epl.v[2], epl.v[2]);
-        xl.p = wave_shift_t<0x138>(epl.p, epl.p);
-        xl.e = wave_shift_t<0x138>(epl.e, epl.e);
+        xl.d = lane_below(epl.d);
+        xl.v[0] = lane_below(epl.v[0]);
+        xl.v[1] = lane_below(epl.v[1]);
+        xl.v[2] = lane_below(epl.v[2]);
+        xl.p = lane_below(epl.p);
+        xl.e = lane_below(epl.e);
         xl.g = gamS;
         if (ej == 5) {
             S F[5];
