@@ -382,7 +382,7 @@ def adjoint_cost(n: int, nsteps: int, every: int):
                           "adjoint_over_forward": t_bwd0 / t_fwd, "segments_recomputed": rec0,
                           "same_gradient_bits": bool(torch.equal(g_rho, g_rho0) and torch.equal(g_p, g_p0))},
             "note": "checkpoints every 20 steps; the forward also keeps the per-step states of as many "
-                    "segments as half the free HBM holds (all of them here), so backward = reverse sweep; "
+                    "segments as half the free HBM holds (segments_recomputed says how many it could not), "
                     "'recompute' keeps one segment and recomputes the others from their checkpoints; "
                     "forward_ms is the same frozen-dt map without checkpoints (no per-step read-back), "
                     "forward_courant_ms the forward that forms each step's Courant dt on the device (with checkpoints)",
@@ -556,6 +556,9 @@ def main() -> int:
     ap.add_argument("--eos-only", action="store_true", help="run only the tabulated-EOS line")
     ap.add_argument("--options-only", action="store_true", help="run only the flux_ang / gravity line")
     ap.add_argument("--adjoint-size", type=int, default=128)
+    ap.add_argument("--adjoint-large", type=int, default=256,
+                    help="a second adjoint line at this size, where the segment stores cannot hold every "
+                         "segment (0: none)")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--mixed-study", action="store_true",
                     help="mixed form vs FP64 at --size for --steps steps: speed and per-field error")
@@ -592,7 +595,10 @@ def main() -> int:
         print(json.dumps({"options": options_cost(args.size, 3)}))
         return 0
     if args.adjoint_only:
-        print(json.dumps({"adjoint": adjoint_cost(args.adjoint_size, args.adjoint_steps, 20)}))
+        res = {"adjoint": adjoint_cost(args.adjoint_size, args.adjoint_steps, 20)}
+        if args.adjoint_large:
+            res["adjoint_256"] = adjoint_cost(args.adjoint_large, 200, 20)
+        print(json.dumps(res))
         return 0
     if args.graph_only:
         print(json.dumps({"graph": graph_cost(600, 50, args.size)}))
@@ -699,6 +705,9 @@ def main() -> int:
     if not args.no_adjoint:
         log(f"adjoint {args.adjoint_size}^3 x {args.adjoint_steps} ...")
         out["adjoint"] = adjoint_cost(args.adjoint_size, args.adjoint_steps, 20)
+        if args.adjoint_large:
+            log(f"adjoint {args.adjoint_large}^3 x 200 ...")
+            out["adjoint_256"] = adjoint_cost(args.adjoint_large, 200, 20)
     if not args.no_cpu:
         log(f"cpu baseline ({args.cpu_steps} steps, {args.cpu_threads} threads) ...")
         out["cpu_baseline"] = cpu_baseline(prm, ic, args.cpu_steps, args.cpu_threads)
