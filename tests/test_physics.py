@@ -180,3 +180,43 @@ def test_reference_setup_overlays_shipped_sodtest(golden):
     l1_ex = np.sum(np.abs(prof[inner] - ex[:, 0])) * 0.01
     assert l1_old < 2e-2, l1_old
     assert l1_ex < 1.2e-2, l1_ex
+
+
+def test_oracle_diagnostics_closed_forms():
+    """The oracle's per-step diagnostics (conservation_diagnostic, utils.f90:1021-1060,
+    run at main.f90:222-223 by the CPU baseline with diagnostics) on states
+    whose sums have closed forms: a uniform state (mass, momenta, energies;
+    centre of mass and angular momentum zero by symmetry) and a rigid rotation
+    about z with a density ramp in x (angular momentum sum of rho omega r^2 dV,
+    x centre of mass sum of x rho dV)."""
+    prm = HydroParams(nx=14, ny=12, nz=10, dx=0.5, dy=0.25, dz=2.0)
+    n, shp = prm.ngrid, prm.shape
+    dv = prm.dx * prm.dy * prm.dz
+    x, y, z = ics.cell_centres(prm)
+    interior = ics.boundary_flags(prm) == 0
+    V = interior.sum() * dv
+
+    def grid(rho, m, E):
+        ic = {"cons": np.zeros((n, 5)), "u": np.zeros((n, 3)), "p": np.ones(n), "adindx": np.full(n, 1.4)}
+        ic["cons"][:, 0] = np.broadcast_to(rho, shp).reshape(n)
+        for k in range(3):
+            ic["cons"][:, 1 + k] = np.broadcast_to(m[k], shp).reshape(n)
+            ic["u"][:, k] = ic["cons"][:, 1 + k] / ic["cons"][:, 0]
+        ic["cons"][:, 4] = np.broadcast_to(E, shp).reshape(n)
+        og = po.OracleGrid(prm, ic)
+        return og.diagnostics()
+
+    d = grid(2.0, (1.0, -0.5, 0.25), 3.0)
+    # mass, momx, momy, momz, amom, etot, ekin, xcom, ycom, zcom, em, ef
+    want = [2 * V, 1 * V, -0.5 * V, 0.25 * V, 0.0, 3 * V, (1 + 0.25 + 0.0625) / 4 * V, 0.0, 0.0, 0.0, 2 * V, 0.0]
+    for k, (a, b) in enumerate(zip(d, want)):
+        assert a == pytest.approx(b, rel=1e-12, abs=1e-12 * V), k
+
+    om = 0.3
+    X, Y = np.broadcast_to(x[None, None, :], shp), np.broadcast_to(y[None, :, None], shp)
+    rho = 1.0 + 0.1 * X
+    d = grid(rho, (-om * Y * rho, om * X * rho, 0.0 * X), 1.0)
+    r2 = (X * X + Y * Y)[interior]
+    assert d[4] == pytest.approx(float(np.sum(om * rho[interior] * r2)) * dv, rel=1e-12)
+    assert d[7] == pytest.approx(float(np.sum(X[interior] * rho[interior])) * dv, rel=1e-12)
+    assert d[0] == pytest.approx(float(np.sum(rho[interior])) * dv, rel=1e-12)
