@@ -275,9 +275,24 @@ void fused_defaults(bx_ctx *c)
             c->fused_lz = lz;
             c->fused_occ = 10;
         } else {
-            int lz = 64;
+            // One workgroup per CU at a time (LDS): the chunk that minimises
+            // the rounds of workgroups times the planes each one marches
+            // (its chunk and the 4 halo planes); ties go to the smaller chunk
+            int ncu = 0;
+            if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+                ncu < 1)
+                ncu = 256;
             const int64_t tiles = (int64_t)((c->nx - 4 + 63) / 64) * ((c->ny - 4 + 7) / 8);
-            while (lz > 4 && tiles * ((nzi + lz - 1) / lz) < 512) lz /= 2;
+            int lz = 4;
+            int64_t best = INT64_MAX;
+            for (int cand = 4; cand <= 256; cand *= 2) {
+                const int64_t chunks = (nzi + cand - 1) / cand;
+                const int64_t cost = (tiles * chunks + ncu - 1) / ncu * ((cand < nzi ? cand : nzi) + 4);
+                if (cost < best) {
+                    best = cost;
+                    lz = cand;
+                }
+            }
             c->fused_lz = lz;
             c->fused_occ = 11;
         }
