@@ -418,6 +418,38 @@ def walls_cost(n: int, steps: int, warmup: int, outflow_ms: float):
             "fallback_steps": fb, "path_of_3_fused_mode_steps": {"fused": fused, "exact": exact}}
 
 
+def host_cost(prm, ic, steps: int):
+    """The step with the boundary's host hand-over included (PCIe): upload of
+    the reference's host arrays (cons, u, p, adindx), `steps` graph-mode
+    steps, download of cons, u and p.  Reported beside `value`, never as it."""
+    import numpy as np
+
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+
+    g = HydroGrid(prm, ic, device=0)
+    g.run(3, mode=nv.BX_STEP_GRAPH, endtime=1e300)
+    g.sync()
+    t0 = time.perf_counter()
+    g.upload(ic)
+    g.sync()
+    t1 = time.perf_counter()
+    g.run(steps, mode=nv.BX_STEP_GRAPH, endtime=1e300)
+    g.sync()
+    t2 = time.perf_counter()
+    g.download()
+    t3 = time.perf_counter()
+    g.close()
+    up_b = sum(8 * np.asarray(ic[k]).size for k in ("cons", "u", "p", "adindx") if ic.get(k) is not None)
+    down_b = 8 * prm.ngrid * 9
+    return {"workload": f"the headline box, {steps} graph-mode steps between an upload and a download",
+            "upload_s": t1 - t0, "upload_GBps": up_b / (t1 - t0) / 1e9,
+            "download_s": t3 - t2, "download_GBps": down_b / (t3 - t2) / 1e9,
+            "steps_s": t2 - t1,
+            "cell_updates_per_s_with_transfers": prm.interior_cells * steps / (t3 - t0),
+            "note": "host arrays in the reference's layout (pageable memory); the library reorders on the device"}
+
+
 def eos_cost(n: int, steps: int, reps: int):
     """The tabulated H2 EOS (H2STAT=0, the reference's default 10000 x 1000
     tables) on an n^3 molecular-cloud box: host table build, the state()
@@ -552,6 +584,7 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward cost")
     ap.add_argument("--no-eos", action="store_true", help="skip the tabulated-EOS line")
     ap.add_argument("--no-walls", action="store_true", help="skip the all-walls box line")
+    ap.add_argument("--no-host", action="store_true", help="skip the line with host transfers")
     ap.add_argument("--no-options", action="store_true", help="skip the flux_ang / gravity line")
     ap.add_argument("--eos-only", action="store_true", help="run only the tabulated-EOS line")
     ap.add_argument("--options-only", action="store_true", help="run only the flux_ang / gravity line")
@@ -680,6 +713,9 @@ def main() -> int:
     if not args.no_walls:
         log("all-walls box ...")
         out["walls"] = walls_cost(args.size, args.steps, args.warmup, ms_step)
+    if not args.no_host:
+        log("with host transfers ...")
+        out["with_host_transfers"] = host_cost(prm, ic, args.steps)
     if not args.no_mixed:
         msz = args.mixed_size
         log(f"mixed form at {msz}^3: {args.mixed_steps} steps each precision ...")
