@@ -425,7 +425,14 @@ __device__ __forceinline__ void tile_stage(const StageArgs &a, TileLds<S, TY> &L
     if (g.kb >= g.ke) return;  // uniform over the workgroup
     unsigned int flg = 0;
     double rmax = 0.0;
-    if (g.r >= 0 && g.r < TY) tile_owner<STAGE, S, TY>(a, L, g, flg, rmax);
+    if (g.r >= 0 && g.r < TY) {
+        // The owner rows are the critical path between the barriers; each SIMD
+        // holds two of them and one halo or edge wave, which is older and would
+        // win the issue arbitration by age: the owners issue first
+        // (2.06 -> 2.00 ms per step at 256^3).
+        __builtin_amdgcn_s_setprio(1);
+        tile_owner<STAGE, S, TY>(a, L, g, flg, rmax);
+    }
     else if (g.r == TY + 1) tile_edge<STAGE, S, TY>(a, L, g, flg);
     else tile_halo<STAGE, S, TY>(a, L, g, flg);
     if (flg) atomicOr(a.flags, flg);
