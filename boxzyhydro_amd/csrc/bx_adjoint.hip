@@ -372,6 +372,141 @@ __global__ __launch_bounds__(256) void k_adj_march(StageArgs a, const double *__
     if (xin && s1 >= jb && s1 < je && s1 == nd - 2) add_to(LW, n, U(s1), xoff, acc0, d == 1 || ghost(s1));
 }
 
+// ------------------------------------------------------- pointwise maps
+// Ghost gather: the ghosts whose donor is cell c (inverse of the ghost table),
+// slot[c] = -1 when none.  Each ghost's contribution is evaluated at the
+// ghost's own values and added with its momentum signs.
+struct GhostInv {
+    const int32_t *slot, *start, *gidx, *gsign;
+};
+
+__device__ __forceinline__ void add_signed(double acc[5], const double v[5], int sg)
+{
+    acc[0] += v[0];
+    for (int f = 1; f <= 3; f++) acc[f] += ((sg >> (f - 1)) & 1) ? -v[f] : v[f];
+    acc[4] += v[4];
+}
+
+// stage 2 at one cell: primitive adjoints lw of W* = (rho*, m*/rho*, p*, e*)
+// at U* = (r, m, E) -> adjoint of U* (out[0..4]) and of p* (out[5])
+__device__ __forceinline__ void fin2_core(double r, const double m[3], double E, const double lw[6], double seps,
+                                          double out[6])
+{
+    const double u[3] = {m[0] / r, m[1] / r, m[2] / r};
+    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const bool eact = E - 0.5 * r * uu > seps;
+    const double le = lw[5];
+    double lr = lw[0] + (eact ? -0.5 * uu * le : 0.0);
+    const double ir = arcp(r);
+    for (int j = 0; j < 3; j++) {
+        const double lu = lw[1 + j] + (eact ? -r * u[j] * le : 0.0);
+        out[1 + j] = lu * ir;
+        lr -= lu * u[j] * ir;
+    }
+    out[0] = lr;
+    out[4] = eact ? le : 0.0;
+    out[5] = lw[4];
+}
+
+__device__ __forceinline__ void fin2_cell(int64_t n, int64_t c, double seps, const double *__restrict__ us,
+                                          const double *__restrict__ LW, double out[6])
+{
+    const double m[3] = {us[n + c], us[2 * n + c], us[3 * n + c]};
+    const double lw[6] = {LW[c], LW[n + c], LW[2 * n + c], LW[3 * n + c], LW[4 * n + c], LW[5 * n + c]};
+    fin2_core(us[c], m, us[4 * n + c], lw, seps, out);
+}
+
+// the rest of stage 2's reverse at an interior cell, from a[6] (the W*
+// reverse of the cell and of the ghosts it donates to) and hl = lam/2: the
+// direct halves of the corrector, and the reverse of
+// p* = max(E* - rho* |u'|^2/2, eps)(gamma-1) with the stage-1 velocity
+// u' = m'/rho' (main.f90:353); U^n = (r0, m0) and U* = (r1, ., E1).  Gives the
+// adjoint of U* (lus) and the sum of that and the adjoint of
+// U' = source(0)(U^n) (lsum), the two terms stage 1's reverse adds.
+__device__ __forceinline__ void mid2_tail(double a[6], const double hl[5], double r0, const double m0[3], double r1,
+                                          double E1, double gm1, double seps, double lus[5], double lsum[5])
+{
+    for (int f = 0; f < 5; f++) a[f] += hl[f];
+    const double w[3] = {m0[0] / r0, m0[1] / r0, m0[2] / r0};
+    const double ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+    const double lp = a[5];
+    double lw[3] = {0.0, 0.0, 0.0};
+    if (E1 - r1 * 0.5 * ww > seps) {
+        a[4] += gm1 * lp;
+        a[0] -= gm1 * lp * 0.5 * ww;
+        for (int j = 0; j < 3; j++) lw[j] = -gm1 * lp * r1 * w[j];
+    }
+    double lup[5];
+    const double ir0 = arcp(r0);
+    double lr0 = 0.0;
+    for (int j = 0; j < 3; j++) {
+        lup[1 + j] = hl[1 + j] + lw[j] * ir0;
+        lr0 -= lw[j] * w[j] * ir0;
+    }
+    lup[0] = hl[0] + lr0;
+    lup[4] = hl[4];
+    for (int f = 0; f < 5; f++) {
+        lus[f] = a[f];
+        lsum[f] = a[f] + lup[f];
+    }
+}
+
+// stage 1 at one cell: primitive adjoints lw of W = prims(source(0)(U^n))
+// at U^n = (r, m, E) plus the direct terms ls (U* = U' + fluxtmp; the
+// corrector's U'; zero on ghosts), through the re-round to the adjoint of U^n
+__device__ __forceinline__ void fin1_core(double r, const double m[3], double E, const double lw[6],
+                                          const double ls[5], double gm1, double seps, double out[5])
+{
+    const double ek = 0.5 * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) / r;
+    const bool ract = E - ek > seps;
+    const double E1 = vmax(E - ek, seps) + ek;
+    const double u[3] = {m[0] / r, m[1] / r, m[2] / r};
+    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const bool eact = E1 - 0.5 * r * uu > seps;
+    const double le = lw[5] + gm1 * lw[4];  // p = e (gamma-1)
+    double lr = lw[0] + (eact ? -0.5 * uu * le : 0.0);
+    double lE1 = eact ? le : 0.0;
+    double lm[3];
+    const double ir = arcp(r);
+    for (int j = 0; j < 3; j++) {
+        const double lu = lw[1 + j] + (eact ? -r * u[j] * le : 0.0);
+        lm[j] = lu * ir;
+        lr -= lu * u[j] * ir;
+    }
+    lr += ls[0];
+    for (int j = 0; j < 3; j++) lm[j] += ls[1 + j];
+    lE1 += ls[4];
+    // E1 = max(E - ek, eps) + ek
+    double lE = lE1;
+    if (!ract) {
+        lE = 0.0;
+        for (int j = 0; j < 3; j++) lm[j] += lE1 * m[j] * ir;
+        lr -= lE1 * ek * ir;
+    }
+    out[0] = lr;
+    for (int j = 0; j < 3; j++) out[1 + j] = lm[j];
+    out[4] = lE;
+}
+
+__device__ __forceinline__ void fin1_cell(int64_t n, int64_t c, double gm1, double seps, const double *__restrict__ u0,
+                                          const double *__restrict__ lsum, const double *__restrict__ LW, double out[5])
+{
+    const double m[3] = {u0[n + c], u0[2 * n + c], u0[3 * n + c]};
+    const double lw[6] = {LW[c], LW[n + c], LW[2 * n + c], LW[3 * n + c], LW[4 * n + c], LW[5 * n + c]};
+    const double ls[5] = {lsum[c], lsum[n + c], lsum[2 * n + c], lsum[3 * n + c], lsum[4 * n + c]};
+    fin1_core(u0[c], m, u0[4 * n + c], lw, ls, gm1, seps, out);
+}
+
+// Where the x kernel ends a stage's reverse for the interior cells that
+// donate to no ghost (the donors gather their ghosts' adjoints in the
+// pointwise kernels below, after every direction): stage 2 writes lus and
+// lsum, stage 1 the adjoint of U^n.
+struct AdjTail {
+    const int32_t *slot;  // per cell: its donor entry, or -1
+    double *lus, *lsum;   // stage 2: outputs; stage 1: lsum is an input
+    double *lout;         // stage 1: output
+};
+
 // ----------------------------------------------------- direction kernel (x)
 // A wave holds 64 consecutive cells of the row-major x-y plane of one z-plane
 // (as the forward x/z kernel; ghost columns separate rows and carry zero
@@ -381,7 +516,7 @@ constexpr int kAdjOut = 58;
 
 template <int STAGE>
 __global__ __launch_bounds__(256) void k_adj_x(StageArgs a, const double *__restrict__ lam, double gscale,
-                                               double kappa, double *__restrict__ LW)
+                                               double kappa, double *__restrict__ LW, AdjTail tail)
 {
     const int lane = threadIdx.x & 63;
     const int w = wave_id();
@@ -450,160 +585,87 @@ __global__ __launch_bounds__(256) void k_adj_x(StageArgs a, const double *__rest
     gc.e += fromL.e + fromR.e;
     if (emit) {
         const double v[6] = {gc.d, gc.u0, gc.u1, gc.u2, gc.p, gc.e};
-        for (int k = 0; k < 6; k++) elem<double>(LW, (int64_t)k * n + zb, coff) = assign ? v[k] : prev[k] + v[k];
+        double lw[6];
+        for (int k = 0; k < 6; k++) lw[k] = assign ? v[k] : prev[k] + v[k];
+        if (ghost || tail.slot[zb + F] >= 0) {
+            for (int k = 0; k < 6; k++) elem<double>(LW, (int64_t)k * n + zb, coff) = lw[k];
+        } else {
+            // an interior cell that donates to no ghost: the rest of the stage
+            const double *U = (const double *)a.uin;
+            const double r = elem<double>(U, zb, coff), E = elem<double>(U, 4 * n + zb, coff);
+            const double m[3] = {elem<double>(U, n + zb, coff), elem<double>(U, 2 * n + zb, coff),
+                                 elem<double>(U, 3 * n + zb, coff)};
+            if constexpr (STAGE == 2) {
+                double out[6], ls[5], lsum[5];
+                fin2_core(r, m, E, lw, a.seps, out);
+                const double *U0 = (const double *)a.uold;
+                const double m0[3] = {elem<double>(U0, n + zb, coff), elem<double>(U0, 2 * n + zb, coff),
+                                      elem<double>(U0, 3 * n + zb, coff)};
+                double hl[5];  // lam/2, loaded again rather than held through the reverse
+                for (int f = 0; f < 5; f++) hl[f] = gscale * elem<double>(lam, (int64_t)f * n + zb, coff);
+                mid2_tail(out, hl, elem<double>(U0, zb, coff), m0, r, E, a.gm1, a.seps, ls, lsum);
+                for (int f = 0; f < 5; f++) {
+                    elem<double>(tail.lus, (int64_t)f * n + zb, coff) = ls[f];
+                    elem<double>(tail.lsum, (int64_t)f * n + zb, coff) = lsum[f];
+                }
+            } else {
+                double ls[5], out[5];
+                for (int f = 0; f < 5; f++) ls[f] = elem<double>(tail.lsum, (int64_t)f * n + zb, coff);
+                fin1_core(r, m, E, lw, ls, a.gm1, a.seps, out);
+                for (int f = 0; f < 5; f++) elem<double>(tail.lout, (int64_t)f * n + zb, coff) = out[f];
+            }
+        }
     }
 }
 
 // ------------------------------------------------------- pointwise kernels
-__device__ __forceinline__ bool cell_interior(int64_t c, int nx, int ny, int nz, bool flatz)
-{
-    const int64_t plane = (int64_t)nx * ny;
-    const int z = (int)(c / plane);
-    const int64_t r = c - (int64_t)z * plane;
-    const int y = (int)(r / nx), x = (int)(r - (int64_t)y * nx);
-    return x >= 2 && x <= nx - 3 && y >= 2 && y <= ny - 3 && (flatz || (z >= 2 && z <= nz - 3));
-}
-
-// Ghost gather: the ghosts whose donor is cell c (inverse of the ghost table),
-// slot[c] = -1 when none.  Each ghost's contribution is evaluated at the
-// ghost's own values and added with its momentum signs.
-struct GhostInv {
-    const int32_t *slot, *start, *gidx, *gsign;
-};
-
-__device__ __forceinline__ void add_signed(double acc[5], const double v[5], int sg)
-{
-    acc[0] += v[0];
-    for (int f = 1; f <= 3; f++) acc[f] += ((sg >> (f - 1)) & 1) ? -v[f] : v[f];
-    acc[4] += v[4];
-}
-
-// stage 2 at one cell: primitive adjoints of W* = (rho*, m*/rho*, p*, e*) ->
-// adjoint of U* (out[0..4]) and of p* (out[5])
-__device__ __forceinline__ void fin2_cell(int64_t n, int64_t c, double seps, const double *__restrict__ us,
-                                          const double *__restrict__ LW, double out[6])
-{
-    const double r = us[c], m[3] = {us[n + c], us[2 * n + c], us[3 * n + c]}, E = us[4 * n + c];
-    const double u[3] = {m[0] / r, m[1] / r, m[2] / r};
-    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    const bool eact = E - 0.5 * r * uu > seps;
-    const double le = LW[5 * n + c];
-    double lr = LW[c] + (eact ? -0.5 * uu * le : 0.0);
-    const double ir = arcp(r);
-    for (int j = 0; j < 3; j++) {
-        const double lu = LW[(1 + j) * n + c] + (eact ? -r * u[j] * le : 0.0);
-        out[1 + j] = lu * ir;
-        lr -= lu * u[j] * ir;
-    }
-    out[0] = lr;
-    out[4] = eact ? le : 0.0;
-    out[5] = LW[4 * n + c];
-}
-
-// interior cells, after the stage-2 directions: adjoint of U* (lus) and of
-// U' = source(0)(U^n) (lup) -- W* reverse at the cell and at the ghosts it
-// donates to, the direct halves of the corrector, and the reverse of
-// p* = max(E* - rho* |u'|^2/2, eps)(gamma-1) with the stage-1 velocity
-// u' = m'/rho' (main.f90:353).  Ghost entries of lus and lup stay zero.
-__global__ __launch_bounds__(256) void k_adj_mid2(int64_t n, int nx, int ny, int nz, int flatz, double gm1,
-                                                 double seps, const double *__restrict__ u0,
+// The donors (interior cells some ghost copies), after the stage-2
+// directions: their own W* reverse plus their ghosts', then mid2_tail.
+// Ghost entries of lus and lsum stay zero.
+__global__ __launch_bounds__(256) void k_adj_mid2(int64_t n, int ndonor, const int32_t *__restrict__ donor,
+                                                 double gm1, double seps, const double *__restrict__ u0,
                                                  const double *__restrict__ us, const double *__restrict__ lam,
                                                  const double *__restrict__ LW, GhostInv gi,
-                                                 double *__restrict__ lus, double *__restrict__ lup)
+                                                 double *__restrict__ lus, double *__restrict__ lsum)
 {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n || !cell_interior(c, nx, ny, nz, flatz)) return;
+    const int kd = blockIdx.x * blockDim.x + threadIdx.x;
+    if (kd >= ndonor) return;
+    const int64_t c = donor[kd];
     double a[6];
     fin2_cell(n, c, seps, us, LW, a);
-    const int k = gi.slot[c];
-    if (k >= 0)
-        for (int t = gi.start[k]; t < gi.start[k + 1]; t++) {
-            double b[6];
-            fin2_cell(n, gi.gidx[t], seps, us, LW, b);
-            add_signed(a, b, gi.gsign[t]);
-            a[5] += b[5];
-        }
-    for (int f = 0; f < 5; f++) a[f] += 0.5 * lam[f * n + c];
-    // p* reverse
-    const double r0 = u0[c];
-    const double w[3] = {u0[n + c] / r0, u0[2 * n + c] / r0, u0[3 * n + c] / r0};
-    const double r1 = us[c], E1 = us[4 * n + c];
-    const double ww = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
-    const double lp = a[5];
-    double lw[3] = {0.0, 0.0, 0.0};
-    if (E1 - r1 * 0.5 * ww > seps) {
-        a[4] += gm1 * lp;
-        a[0] -= gm1 * lp * 0.5 * ww;
-        for (int j = 0; j < 3; j++) lw[j] = -gm1 * lp * r1 * w[j];
+    for (int t = gi.start[kd]; t < gi.start[kd + 1]; t++) {
+        double b[6];
+        fin2_cell(n, gi.gidx[t], seps, us, LW, b);
+        add_signed(a, b, gi.gsign[t]);
+        a[5] += b[5];
     }
-    for (int f = 0; f < 5; f++) lus[f * n + c] = a[f];
-    const double ir0 = arcp(r0);
-    double lr0 = 0.0;
-    for (int j = 0; j < 3; j++) {
-        lup[(1 + j) * n + c] = 0.5 * lam[(1 + j) * n + c] + lw[j] * ir0;
-        lr0 -= lw[j] * w[j] * ir0;
+    double hl[5], ls[5], lsm[5];
+    for (int f = 0; f < 5; f++) hl[f] = 0.5 * lam[f * n + c];
+    const double m0[3] = {u0[n + c], u0[2 * n + c], u0[3 * n + c]};
+    mid2_tail(a, hl, u0[c], m0, us[c], us[4 * n + c], gm1, seps, ls, lsm);
+    for (int f = 0; f < 5; f++) {
+        lus[f * n + c] = ls[f];
+        lsum[f * n + c] = lsm[f];
     }
-    lup[c] = 0.5 * lam[c] + lr0;
-    lup[4 * n + c] = 0.5 * lam[4 * n + c];
 }
 
-// stage 1 at one cell: primitive adjoints of W = prims(source(0)(U^n)) plus
-// the direct terms (U* = U' + fluxtmp; the corrector's U'; zero on ghosts),
-// through the re-round to the adjoint of U^n
-__device__ __forceinline__ void fin1_cell(int64_t n, int64_t c, double gm1, double seps, const double *__restrict__ u0,
-                                          const double *__restrict__ lus, const double *__restrict__ lup,
-                                          const double *__restrict__ LW, double out[5])
+// The donors after the stage-1 directions: the adjoint of U^n at the cell
+// plus that of the ghosts it donates to (ghost entries of lout stay zero).
+__global__ __launch_bounds__(256) void k_adj_fin1(int64_t n, int ndonor, const int32_t *__restrict__ donor,
+                                                 double gm1, double seps, const double *__restrict__ u0,
+                                                 const double *__restrict__ lsum, const double *__restrict__ LW,
+                                                 GhostInv gi, double *__restrict__ lout)
 {
-    const double r = u0[c], m[3] = {u0[n + c], u0[2 * n + c], u0[3 * n + c]}, E = u0[4 * n + c];
-    const double ek = 0.5 * (m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) / r;
-    const bool ract = E - ek > seps;
-    const double E1 = vmax(E - ek, seps) + ek;
-    const double u[3] = {m[0] / r, m[1] / r, m[2] / r};
-    const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-    const bool eact = E1 - 0.5 * r * uu > seps;
-    const double le = LW[5 * n + c] + gm1 * LW[4 * n + c];  // p = e (gamma-1)
-    double lr = LW[c] + (eact ? -0.5 * uu * le : 0.0);
-    double lE1 = eact ? le : 0.0;
-    double lm[3];
-    const double ir = arcp(r);
-    for (int j = 0; j < 3; j++) {
-        const double lu = LW[(1 + j) * n + c] + (eact ? -r * u[j] * le : 0.0);
-        lm[j] = lu * ir;
-        lr -= lu * u[j] * ir;
-    }
-    lr += lus[c] + lup[c];
-    for (int j = 0; j < 3; j++) lm[j] += lus[(1 + j) * n + c] + lup[(1 + j) * n + c];
-    lE1 += lus[4 * n + c] + lup[4 * n + c];
-    // E1 = max(E - ek, eps) + ek
-    double lE = lE1;
-    if (!ract) {
-        lE = 0.0;
-        for (int j = 0; j < 3; j++) lm[j] += lE1 * m[j] * ir;
-        lr -= lE1 * ek * ir;
-    }
-    out[0] = lr;
-    for (int j = 0; j < 3; j++) out[1 + j] = lm[j];
-    out[4] = lE;
-}
-
-// interior cells, after the stage-1 directions: adjoint of U^n at the cell
-// plus that of the ghosts it donates to (ghost entries of lout stay zero)
-__global__ __launch_bounds__(256) void k_adj_fin1(int64_t n, int nx, int ny, int nz, int flatz, double gm1,
-                                                 double seps, const double *__restrict__ u0,
-                                                 const double *__restrict__ lus, const double *__restrict__ lup,
-                                                 const double *__restrict__ LW, GhostInv gi, double *__restrict__ lout)
-{
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n || !cell_interior(c, nx, ny, nz, flatz)) return;
+    const int kd = blockIdx.x * blockDim.x + threadIdx.x;
+    if (kd >= ndonor) return;
+    const int64_t c = donor[kd];
     double a[5];
-    fin1_cell(n, c, gm1, seps, u0, lus, lup, LW, a);
-    const int k = gi.slot[c];
-    if (k >= 0)
-        for (int t = gi.start[k]; t < gi.start[k + 1]; t++) {
-            double b[5];
-            fin1_cell(n, gi.gidx[t], gm1, seps, u0, lus, lup, LW, b);
-            add_signed(a, b, gi.gsign[t]);
-        }
+    fin1_cell(n, c, gm1, seps, u0, lsum, LW, a);
+    for (int t = gi.start[kd]; t < gi.start[kd + 1]; t++) {
+        double b[5];
+        fin1_cell(n, gi.gidx[t], gm1, seps, u0, lsum, LW, b);
+        add_signed(a, b, gi.gsign[t]);
+    }
     for (int f = 0; f < 5; f++) lout[f * n + c] = a[f];
 }
 
@@ -651,7 +713,7 @@ struct bx_adj {
     int64_t store_budget = -1;  // bytes for segment stores; -1: half the free device memory
     double *lam[2] = {nullptr, nullptr};
     double *lus = nullptr;  // 5 planes: adjoint of U* (ghost entries zero)
-    double *lup = nullptr;  // 5 planes: adjoint of U' (ghost entries zero)
+    double *lsum = nullptr;  // 5 planes: adjoint of U* plus that of U' (ghost entries zero)
     double *lw = nullptr;   // 6 planes: primitive adjoints
     int32_t *fold = nullptr;  // donor, start, ghost, sign arrays (one allocation)
     int32_t *slot = nullptr;  // per cell: index of its donor entry, or -1
@@ -668,7 +730,7 @@ void adjoint_free(bx_ctx *c)
 {
     if (!c->adj) return;
     bx_adj *j = c->adj;
-    void *p[] = {j->ckpt, j->lam[0], j->lam[1], j->lus, j->lup, j->lw, j->fold, j->slot, j->dlog};
+    void *p[] = {j->ckpt, j->lam[0], j->lam[1], j->lus, j->lsum, j->lw, j->fold, j->slot, j->dlog};
     for (void *q : p)
         if (q) (void)hipFree(q);
     for (double *q : j->stores)
@@ -695,11 +757,11 @@ static int adj_init(bx_ctx *c)
     BX_HIP(hipMalloc((void **)&j->lam[0], 5 * pb));
     BX_HIP(hipMalloc((void **)&j->lam[1], 5 * pb));
     BX_HIP(hipMalloc((void **)&j->lus, 5 * pb));
-    BX_HIP(hipMalloc((void **)&j->lup, 5 * pb));
+    BX_HIP(hipMalloc((void **)&j->lsum, 5 * pb));
     BX_HIP(hipMalloc((void **)&j->lw, 6 * pb));
     // ghost entries of the adjoint planes are never written and stay zero, and
     // so do the primitive adjoints of cells no direction reaches
-    for (double *q : {j->lam[0], j->lam[1], j->lus, j->lup}) BX_HIP(hipMemsetAsync(q, 0, 5 * pb, c->stream));
+    for (double *q : {j->lam[0], j->lam[1], j->lus, j->lsum}) BX_HIP(hipMemsetAsync(q, 0, 5 * pb, c->stream));
     BX_HIP(hipMemsetAsync(j->lw, 0, 6 * pb, c->stream));
     // inverse ghost table: donors with the ghosts that copy them
     const int64_t ng = c->nghost;
@@ -765,7 +827,7 @@ static int fold(bx_ctx *c, double *L5, double *L1)
 
 // adjoint of fluxtmp (gscale*lam) -> primitive adjoints in lw, every direction
 template <int STAGE>
-static int directions(bx_ctx *c, const StageArgs &a, const double *lam, double gscale)
+static int directions(bx_ctx *c, const StageArgs &a, const double *lam, double gscale, const AdjTail &tail)
 {
     bx_adj *j = c->adj;
     const double kx = a.ayz * a.dt / a.vol, ky = a.axz * a.dt / a.vol, kz = a.axy * a.dt / a.vol;
@@ -787,7 +849,7 @@ static int directions(bx_ctx *c, const StageArgs &a, const double *lam, double g
         const int64_t span = (int64_t)(c->ny - 4) * c->nx + 6;
         const unsigned waves = (unsigned)((span + kAdjOut - 1) / kAdjOut);
         dim3 grid((waves + 3) / 4, (unsigned)nzi, 1);
-        hipLaunchKernelGGL((k_adj_x<STAGE>), grid, dim3(256), 0, c->stream, a, lam, gscale, kx, j->lw);
+        hipLaunchKernelGGL((k_adj_x<STAGE>), grid, dim3(256), 0, c->stream, a, lam, gscale, kx, j->lw, tail);
         BX_HIP(hipGetLastError());
     }
     return BX_OK;
@@ -811,23 +873,28 @@ static int adj_step(bx_ctx *c, double dt, const double *un, const double *us, co
     bx_adj *j = c->adj;
     int rc;
     const int64_t n = c->n;
-    const unsigned nb = blocks_for(n, 256);
     const GhostInv gi = ghost_inv(c);
-    const int fz = c->flatz ? 1 : 0;
+    const int32_t *donor = j->fold;
+    const unsigned nbd = blocks_for(j->ndonor, 256);
     StageArgs b = base_args(c, dt);
     b.uin = us;
     b.pin = ps;
     b.uold = un;
-    if ((rc = directions<2>(c, b, lam, 0.5))) return rc;
-    hipLaunchKernelGGL(k_adj_mid2, dim3(nb), dim3(256), 0, c->stream, n, c->nx, c->ny, c->nz, fz,
-                       c->prm.gammafix - 1.0, c->prm.small_eps, un, us, lam, j->lw, gi, j->lus, j->lup);
-    BX_HIP(hipGetLastError());
+    // the x kernel ends the reverse of every interior cell but the donors
+    if ((rc = directions<2>(c, b, lam, 0.5, AdjTail{j->slot, j->lus, j->lsum, nullptr}))) return rc;
+    if (j->ndonor) {
+        hipLaunchKernelGGL(k_adj_mid2, dim3(nbd), dim3(256), 0, c->stream, n, j->ndonor, donor,
+                           c->prm.gammafix - 1.0, c->prm.small_eps, un, us, lam, j->lw, gi, j->lus, j->lsum);
+        BX_HIP(hipGetLastError());
+    }
     StageArgs a = base_args(c, dt);
     a.uin = un;
-    if ((rc = directions<1>(c, a, j->lus, 1.0))) return rc;
-    hipLaunchKernelGGL(k_adj_fin1, dim3(nb), dim3(256), 0, c->stream, n, c->nx, c->ny, c->nz, fz,
-                       c->prm.gammafix - 1.0, c->prm.small_eps, un, j->lus, j->lup, j->lw, gi, lout);
-    BX_HIP(hipGetLastError());
+    if ((rc = directions<1>(c, a, j->lus, 1.0, AdjTail{j->slot, nullptr, j->lsum, lout}))) return rc;
+    if (j->ndonor) {
+        hipLaunchKernelGGL(k_adj_fin1, dim3(nbd), dim3(256), 0, c->stream, n, j->ndonor, donor,
+                           c->prm.gammafix - 1.0, c->prm.small_eps, un, j->lsum, j->lw, gi, lout);
+        BX_HIP(hipGetLastError());
+    }
     return BX_OK;
 }
 
