@@ -13,7 +13,10 @@
 //
 // The encoder and decoder are host code on host arrays; bx_write_celldump and
 // bx_read_celldump add the device side: the state() call the reference makes
-// before writing and after reading, on the context's state.
+// before writing and after reading, on the context's state, and the potential
+// column when gravity from a potential is configured (written from and read
+// into the context's phi; zero otherwise, as the reference's phi without a
+// potential).
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -80,17 +83,18 @@ std::string fort_e(double v, int w, int d, int e)
     return out;
 }
 
-}  // namespace
-
-extern "C" int bx_celldump_encode(const bx_params *prm, const char *path, double time, int32_t step, int32_t ascii,
-                                  const double *cons, const double *u, const double *p, const double *muc_array)
+// The writer on host arrays; phi: the potential column (write_files.f90:52,
+// 62), zero where null (no potential configured: phi stays at its set-up
+// zero, init_grid.f90:291, 299).
+int encode(const bx_params *prm, const char *path, double time, int32_t step, int32_t ascii, const double *cons,
+           const double *u, const double *p, const double *muc_array, const double *phi_array)
 {
     if (!prm || !path || !cons || !u || !p) return bx::set_error(BX_ERR_ARG, "bx_celldump_encode: null argument");
     const int nx = prm->nx, ny = prm->ny, nz = prm->nz;
     const int64_t n = (int64_t)nx * ny * nz;
     double scale[8];
     get_units(scale);
-    const double muc = muc_of(*prm), phi = 0.0;
+    const double muc = muc_of(*prm);
     FILE *f = std::fopen(path, ascii ? "w" : "wb");
     if (!f) return bx::set_error(BX_ERR_ARG, std::string("bx_celldump_encode: cannot open ") + path);
     bool ok = true;
@@ -117,7 +121,8 @@ extern "C" int bx_celldump_encode(const bx_params *prm, const char *path, double
         for (int iy = 1; iy <= ny && ok; iy++)
             for (int ix = 1; ix <= nx && ok; ix++, c++) {
                 const double v[11] = {centre(prm->dx, ix, nx), centre(prm->dy, iy, ny), centre(prm->dz, iz, nz),
-                                      cons[c * 5], p[c], u[c * 3], u[c * 3 + 1], u[c * 3 + 2], phi, cons[c * 5 + 4],
+                                      cons[c * 5], p[c], u[c * 3], u[c * 3 + 1], u[c * 3 + 2],
+                                      phi_array ? phi_array[c] : 0.0, cons[c * 5 + 4],
                                       muc_array ? muc_array[c] : muc};
                 const int32_t bd = boundary_of(ix, iy, iz, nx, ny, nz);
                 if (ascii) {
@@ -139,8 +144,10 @@ extern "C" int bx_celldump_encode(const bx_params *prm, const char *path, double
     return 0;
 }
 
-extern "C" int bx_celldump_decode(const bx_params *prm, const char *path, double *time, int32_t *step, double *cons,
-                                  double *u, double *p, double *adindx)
+// The reader on host arrays; phi_out (optional) receives the potential
+// column, which read_hydro.f90:129-131 reads into phi.
+int decode(const bx_params *prm, const char *path, double *time, int32_t *step, double *cons, double *u, double *p,
+           double *adindx, double *phi_out)
 {
     if (!prm || !path || !time || !step || !cons || !u || !p || !adindx)
         return bx::set_error(BX_ERR_ARG, "bx_celldump_decode: null argument");
@@ -177,6 +184,7 @@ extern "C" int bx_celldump_decode(const bx_params *prm, const char *path, double
         u[c * 3 + 1] = v[6];
         u[c * 3 + 2] = v[7];
         cons[c * 5 + 4] = v[9];
+        if (phi_out) phi_out[c] = v[8];
     }
     std::fclose(f);
     // read_hydro.f90:150-178: momenta from u, gamma, then the frame cells
@@ -195,6 +203,20 @@ extern "C" int bx_celldump_decode(const bx_params *prm, const char *path, double
     return 0;
 }
 
+}  // namespace
+
+extern "C" int bx_celldump_encode(const bx_params *prm, const char *path, double time, int32_t step, int32_t ascii,
+                                  const double *cons, const double *u, const double *p, const double *muc_array)
+{
+    return encode(prm, path, time, step, ascii, cons, u, p, muc_array, nullptr);
+}
+
+extern "C" int bx_celldump_decode(const bx_params *prm, const char *path, double *time, int32_t *step, double *cons,
+                                  double *u, double *p, double *adindx)
+{
+    return decode(prm, path, time, step, cons, u, p, adindx, nullptr);
+}
+
 extern "C" int bx_write_celldump(bx_ctx *c, const char *path, double time, int32_t step, int32_t ascii)
 {
     if (!c || !path) return bx::set_error(BX_ERR_ARG, "bx_write_celldump: null argument");
@@ -209,17 +231,25 @@ extern "C" int bx_write_celldump(bx_ctx *c, const char *path, double time, int32
         muc.resize(n);
         if ((rc = bx_download_muc(c, muc.data()))) return rc;
     }
-    return bx_celldump_encode(&c->prm, path, time, step, ascii, cons.data(), u.data(), p.data(),
-                              c->eos ? muc.data() : nullptr);
+    std::vector<double> phi;
+    if (c->grav_flags && c->phi) {  // the configured potential
+        phi.resize(n);
+        BX_HIP(hipMemcpyAsync(phi.data(), c->phi, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        BX_HIP(hipStreamSynchronize(c->stream));
+    }
+    return encode(&c->prm, path, time, step, ascii, cons.data(), u.data(), p.data(), c->eos ? muc.data() : nullptr,
+                  phi.empty() ? nullptr : phi.data());
 }
 
 extern "C" int bx_read_celldump(bx_ctx *c, const char *path, double *time, int32_t *step)
 {
     if (!c || !path || !time || !step) return bx::set_error(BX_ERR_ARG, "bx_read_celldump: null argument");
     const size_t n = (size_t)c->n;
-    std::vector<double> cons(5 * n), u(3 * n), p(n), ad(n);
+    std::vector<double> cons(5 * n), u(3 * n), p(n), ad(n), phi(n);
     int rc;
-    if ((rc = bx_celldump_decode(&c->prm, path, time, step, cons.data(), u.data(), p.data(), ad.data()))) return rc;
+    if ((rc = decode(&c->prm, path, time, step, cons.data(), u.data(), p.data(), ad.data(), phi.data()))) return rc;
     if ((rc = bx_upload(c, cons.data(), u.data(), p.data(), ad.data()))) return rc;
+    // the potential read with the state (read_hydro.f90:129-131), where one is configured
+    if (c->grav_flags && (rc = bx_set_phi(c, phi.data()))) return rc;
     return bx_state(c);  // read_hydro.f90:180
 }

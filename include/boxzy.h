@@ -340,10 +340,13 @@ int bx_cons_device(bx_ctx *ctx, double *planes, int32_t to_ctx);
  * bx_read_celldump replaces read_hydro(): it reads `path`, rebuilds the
  * momenta from u, resets adindx and the frame cells (read_hydro.f90:150-178),
  * uploads, and runs state(); *time and *step receive the file's values.
+ * With gravity from a potential configured (bx_gravity_config), the phi
+ * column is the context's potential and bx_read_celldump sets it from the
+ * file (read_hydro.f90:129-131); otherwise phi is written as zero.
  * The encode/decode pair is the host-only part on host arrays (Fortran
  * layout): encode writes arrays that state() has already updated (muc_array
- * NULL: the fixed-gamma muc of eos.f90:144 in every cell); decode returns the
- * arrays read_hydro holds before its final state() call.
+ * NULL: the fixed-gamma muc of eos.f90:144 in every cell; phi zero); decode
+ * returns the arrays read_hydro holds before its final state() call.
  */
 int bx_write_celldump(bx_ctx *ctx, const char *path, double time, int32_t step, int32_t ascii);
 int bx_read_celldump(bx_ctx *ctx, const char *path, double *time, int32_t *step);

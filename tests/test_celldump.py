@@ -163,3 +163,42 @@ def test_restart_continues_close_to_uninterrupted_run(golden, tmp_path):
     for k in ("cons", "p"):
         a, b = getattr(second, k)[inner], getattr(whole, k)[inner]
         np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.gpu
+def test_gpu_celldump_carries_the_potential(tmp_path):
+    """With gravity from a potential configured, the phi column holds the
+    context's potential (write_files.f90:52,62), and a restart reads it back
+    (read_hydro.f90:129-131): the restarted context forms the same gforce."""
+    from boxzyhydro_amd import ics
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+    from boxzyhydro_amd.params import HydroParams
+
+    prm = HydroParams(nx=12, ny=10, nz=9, dx=0.5, dy=0.25, dz=0.5, endtime=1e30)
+    x, y, z = ics.cell_centres(prm)
+    rng = np.random.default_rng(7)
+    phi = (np.sin(x[None, None, :]) * np.cos(y[None, :, None]) + 0.1 * z[:, None, None]).reshape(-1)
+    phi = phi + 1e-3 * rng.standard_normal(phi.size)
+    ic = ics.box_ic(prm, seed=0)
+    g = HydroGrid(prm, ic, device=0)
+    g.set_gravity(nv.BX_GRAV_POTENTIAL, phi)
+    path = str(tmp_path / "celldump.bin")
+    g.write_celldump(path, 3)
+    raw = open(path, "rb").read()
+    off = (4 + 4 + 4) + (4 + 12 + 4) + (4 + 64 + 4)  # the first three records
+    rec = 4 + 11 * 8 + 4 + 4
+    col = np.array([np.frombuffer(raw, np.float64, 11, off + k * rec + 4)[8] for k in range(prm.ngrid)])
+    assert np.array_equal(col, phi)
+    gf = g.download_gforce()
+    g2 = HydroGrid(prm, ic, device=0)
+    g2.set_gravity(nv.BX_GRAV_POTENTIAL, None)  # a zero potential until the restart
+    g2.read_celldump(path)
+    g2.calc_gforce()
+    assert np.array_equal(g2.download_gforce(), gf)
+    # without a potential configured the column stays zero
+    g3 = HydroGrid(prm, ic, device=0)
+    path3 = str(tmp_path / "celldump0.bin")
+    g3.write_celldump(path3, 3)
+    raw3 = open(path3, "rb").read()
+    assert all(np.frombuffer(raw3, np.float64, 11, off + k * rec + 4)[8] == 0.0 for k in range(prm.ngrid))
