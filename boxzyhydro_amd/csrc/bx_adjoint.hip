@@ -23,10 +23,12 @@
 // Work decomposition (gather form, no atomics): per stage, one kernel per
 // direction turns the adjoint of fluxtmp into adjoints of every cell's
 // primitives W -- y and z march along the direction with a register queue
-// (faces, then interface states, then limiter slopes), x works through wave
-// shuffles -- and pointwise kernels map primitive adjoints to conserved ones.
-// Ghost cells collect adjoints like any cell; a fold then adds each ghost's
-// adjoint to its donor (momentum signs flipped at walls) and clears it.
+// (faces, then interface states, then limiter slopes), x works through
+// whole-wave DPP shifts and, running last, maps the complete primitive
+// adjoint of every interior cell that donates to no ghost to the conserved
+// one.  Ghost cells collect adjoints like any cell; two pointwise kernels over
+// the donors map theirs and add each ghost's reverse to its donor (momentum
+// signs flipped at walls).
 #include <algorithm>
 #include <cmath>
 #include <cstring>

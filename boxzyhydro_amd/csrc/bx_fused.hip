@@ -1,4 +1,4 @@
-// bx_fused.hip -- the hydro step as two fused stencil kernels per step.
+// bx_fused.hip -- the hydro step as one fused stencil kernel per RK stage.
 //
 // The reference step (main.f90:211-555) runs ~25 passes over the grid per
 // step.  On a "regular" state -- every ghost cell a (sign-flipped) copy of an
@@ -19,14 +19,18 @@
 // cell that would take an irregular branch raises a flag and the host redoes
 // the step with the exact sequence.
 //
-// Work decomposition (gfx950, wave64), two kernels per stage, every wave
-// independent (no LDS, no block barriers), each face evaluated once:
+// Work decomposition (gfx950, wave64): by default each stage is one kernel,
+// k_stage, a workgroup holding a 64 x TY tile of outputs with its +-2 halo in
+// LDS and marching in z (bx_tile.h).  The tuning variants 0, 1 and 4
+// (bx_tune) run the earlier form, two kernels per stage with every wave
+// independent (no LDS, no block barriers):
 //   * k_ydir: a wave holds 64 interior x-cells of one z-plane and marches in y
 //     with a register queue; it writes the y-term of fluxtmp (FP64 planes).
 //   * k_xz: a wave holds 64 cells of one interior row (lanes 2..61 output,
 //     0,1,62,63 the x-halo; x-neighbours through wave shuffles) and marches in
 //     z with a register queue; it adds the x- and z-terms in the reference's
 //     order and applies the stage update.
+// Both give the same bits.
 #include <cmath>
 
 #include "bx_tile.h"
