@@ -228,23 +228,29 @@ def batched(n_tubes: int, reps: int):
     import numpy as np
 
     from boxzyhydro_amd import ics
-    from boxzyhydro_amd.autotune import tune_batch
+    from boxzyhydro_amd.autotune import tune_batch, tune_gangs
     from boxzyhydro_amd.batch import TubeBatch
 
     tubes = ics.riemann_tubes(n_tubes, seed=0)
     b = TubeBatch(tubes)
     best, res = tune_batch(b, reps=1)
+    single = b.result()  # one workgroup per tube: the bits a gang must reproduce
+    gbest, gres = tune_gangs(b, reps=1)
     b.run(graph=True)
     b.sync()
     ms = min([(b.run(graph=True), b.sync())[1] for _ in range(reps)])
     out = b.result()
+    same = all(np.array_equal(x[k], y[k]) for x, y in zip(out, single) for k in ("cons", "u", "p")) and \
+        all(x["time"] == y["time"] and x["steps"] == y["steps"] and x["status"] == y["status"]
+            for x, y in zip(out, single))
     steps = np.array([r["steps"] for r in out])
     row_cells = np.asarray(b.nx - 4)
     row_updates = float(np.sum(steps * row_cells))
     _, interior3 = b.cells()
     full_updates = float(np.sum(steps * row_cells * 16))  # each tube grid is L x 4 x 4 interior cells
     return {"workload": f"{n_tubes} shock tubes, L log-uniform in [64,4096] (x4x4 interior), gamma 1.4, "
-                        f"t_end 0.1, FP64, one persistent launch replayed from a hipGraph",
+                        f"t_end 0.1, FP64, one persistent launch replayed from a hipGraph, the costliest "
+                        f"tubes each over a gang of workgroups",
             "ms": ms, "steps_min": int(steps.min()), "steps_max": int(steps.max()),
             "row_cell_updates_per_s": row_updates / (ms * 1e-3),
             "grid_equivalent_cell_updates_per_s": full_updates / (ms * 1e-3),
@@ -252,6 +258,9 @@ def batched(n_tubes: int, reps: int):
                                     "computed once, so this is not a throughput (row_cell_updates_per_s is)",
             "tile": {"waves_per_block": best[0], "blocks_per_cu": best[1]},
             "tile_ms": {f"{c[0]}x{c[1]}": t * 1e3 for c, t in res},
+            "gangs": {"members": gbest[0], "tubes": gbest[1], "fallbacks": b.gang_fallbacks(),
+                      "same_bits_as_one_workgroup_per_tube": bool(same),
+                      "candidates_ms": {f"{c[0]}x{c[1]}": t * 1e3 for c, t in gres}},
             "longest_tube_steps": int(steps.max()),
             "status_bits": int(np.bitwise_or.reduce([r["status"] for r in out]))}
 

@@ -60,6 +60,36 @@ def tune_batch(batch, candidates: Iterable[Tuple[int, int]] = BATCH_CANDIDATES, 
     return best, results
 
 
+# (members per gang, gangs): each of the costliest tubes over several workgroups
+GANG_CANDIDATES = [(0, 0), (4, 16), (4, 32), (4, 48), (6, 24), (6, 32), (8, 16)]
+
+
+def tune_gangs(batch, candidates: Iterable[Tuple[int, int]] = GANG_CANDIDATES, reps: int = 2,
+               graph: bool = True) -> Tuple[Tuple[int, int], List[Tuple[Tuple[int, int], float]]]:
+    """Gangs of the batched-tube launch (bx_batch_gang): with the work split
+    already chosen, each candidate runs the whole batch ``reps`` times (the
+    first run also measures each tube's steps, which pick the gang tubes); the
+    fastest is kept.  Candidates the device cannot hold (members x gangs above
+    its compute units) are skipped.  Results do not depend on the choice."""
+    batch.gang(0, 0)
+    batch.run(graph=graph)
+    batch.sync()  # the measured steps of every tube
+    results = []
+    for cand in candidates:
+        try:
+            batch.gang(*cand)
+        except nv.BoxzyError:
+            continue
+        best = float("inf")
+        for _ in range(reps):
+            batch.run(graph=graph)
+            best = min(best, batch.sync())
+        results.append((cand, best * 1e-3))
+    best = min(results, key=lambda r: r[1])[0]
+    batch.gang(*best)
+    return best, results
+
+
 # the 4- and 8-wave builds of the batched-grid kernel hold about 210 registers
 # per lane (8 waves per CU); the 16-wave build is held to 128 and spills
 BOX_CANDIDATES = [(16, 1), (8, 1), (4, 1), (4, 2)]

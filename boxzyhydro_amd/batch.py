@@ -136,6 +136,18 @@ class TubeBatch:
     def tune(self, waves_per_block: int, blocks_per_cu: int) -> None:
         nv.check(nv.lib().bx_batch_tune(self._h, waves_per_block, blocks_per_cu), "bx_batch_tune")
 
+    def gang(self, members: int, ntubes: int) -> None:
+        """Split each of the ``ntubes`` costliest tubes over ``members``
+        workgroups (bx_batch_gang; 0 turns it off).  Results do not change."""
+        nv.check(nv.lib().bx_batch_gang(self._h, members, ntubes), "bx_batch_gang")
+
+    def gang_fallbacks(self) -> int:
+        """Runs repeated without gangs because a gang's members were not
+        resident together."""
+        n = ctypes.c_int32()
+        nv.check(nv.lib().bx_batch_gang_fallbacks(self._h, ctypes.byref(n)), "bx_batch_gang_fallbacks")
+        return n.value
+
     def log_dt(self, maxlog: int) -> None:
         nv.check(nv.lib().bx_batch_log_dt(self._h, maxlog), "bx_batch_log_dt")
         self.maxlog = maxlog

@@ -83,6 +83,12 @@ struct BatchArgs {
     int *steps_out;
     unsigned *status;
     double *dtlog;
+    int nqueue;                     // tubes in the queue (order[0..nqueue))
+    int ngang, gang_p;              // gangs and members per gang (gang_p > 1: gangs on)
+    const int *gang_tube;           // the tube of each gang
+    unsigned *gang_ctr;             // ngang arrival counters
+    unsigned long long *gang_rate;  // 3 per gang
+    unsigned *gang_fail;
 };
 
 // Element c of a plane, addressed as the plane's (workgroup-uniform) base plus
@@ -203,14 +209,330 @@ __device__ __forceinline__ void x_faces(const StageArgs &sa, const W6<double> &q
     face_flux<double, 0, true>(L, mi, Fx);
 }
 
+// ------------------------------------------------------------------ gangs
+// A tube too long for one CU can be split over a gang of P workgroups, each
+// taking a contiguous range of its segments.  The predictor of a member also
+// covers the two cells past each shared edge (the neighbour computes them too,
+// with the same operations on the same values), so only the end-of-step state
+// of the four cells on either side of an edge crosses between members: those
+// are written and read with device-coherent (agent-scope) accesses, and the
+// members meet at one barrier on a device counter per step, after the
+// corrector, with the Courant maximum.  Every other cell stays with its
+// member.  The arithmetic per cell is the single-workgroup form's, and the
+// maximum is exact, so a gang gives the same bits.  A barrier that waits
+// longer than kGangSpins sleeps (members not resident together) marks the
+// batch failed and the member leaves the tube; the host then runs the batch
+// again without gangs.
+constexpr unsigned kGangSpins = 1u << 24;
+
+struct GangCtx {
+    unsigned *ctr;             // arrivals, monotonic within a run
+    unsigned long long *rate;  // three rotating slots for the Courant maximum (bits of a double >= 0)
+    unsigned *fail;
+    int member, p;
+    unsigned epoch;            // barriers passed (the same in every thread)
+    int exch;                  // maxima exchanged
+};
+
+__device__ __forceinline__ double load_c(const double *p)
+{
+    return __longlong_as_double(__hip_atomic_load((const long long *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+__device__ __forceinline__ void store_c(double *p, double v)
+{
+    __hip_atomic_store((long long *)p, __double_as_longlong(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// false: the wait gave up
+__device__ __forceinline__ bool gang_barrier(GangCtx &g, int *s_ok)
+{
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores and atomics are complete
+    __syncthreads();
+    g.epoch++;
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(g.ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned target = g.epoch * (unsigned)g.p;
+        unsigned spins = 0;
+        int ok = 1;
+        while (__hip_atomic_load(g.ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(4);
+            if (++spins > kGangSpins) {
+                ok = 0;
+                __hip_atomic_fetch_or(g.fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+        *s_ok = ok;
+    }
+    __syncthreads();
+    return *s_ok != 0;
+}
+
+// the gang's maximum of each member's r (every thread holds its member's r)
+__device__ __forceinline__ bool gang_max(GangCtx &g, double &r, int *s_ok, double *s_r)
+{
+    const int slot = g.exch % 3;
+    if (threadIdx.x == 0)
+        __hip_atomic_fetch_max(&g.rate[slot], (unsigned long long)__double_as_longlong(r), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    if (!gang_barrier(g, s_ok)) return false;
+    if (threadIdx.x == 0) {
+        *s_r = __longlong_as_double(
+            (long long)__hip_atomic_load(&g.rate[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        // the slot two exchanges on: every member read it before two barriers ago
+        if (g.member == 0)
+            __hip_atomic_store(&g.rate[(slot + 1) % 3], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    r = *s_r;
+    g.exch++;
+    return true;
+}
+
+// One tube ti; GANG: this workgroup is member g.member of g.p, else the whole
+// tube.  Every thread of the workgroup calls it.
+template <int W, bool GANG>
+__device__ __forceinline__ void run_tube(const BatchArgs &a, const Tube &t, int ti, GangCtx &g, double *red, int *s_ok,
+                         double *s_r, unsigned *sflag, double (*ext)[9][2])
+{
+    const int lane = threadIdx.x & 63, wv = wave_id();  // wave-uniform for the compiler
+    const int64_t T = a.total;
+    const int nx = t.nx;
+    const int nseg = (nx - 4 + kSegOut - 1) / kSegOut;
+    // segments [s0, s1) and output cells [c_lo, c_hi) of this workgroup
+    int s0 = 0, s1 = nseg, c_lo = 0, c_hi = nx;
+    if (GANG) {
+        const int per = (nseg + g.p - 1) / g.p;
+        s0 = min(nseg, g.member * per);
+        s1 = min(nseg, s0 + per);
+        c_lo = g.member == 0 ? 0 : s0 * kSegOut + 2;
+        c_hi = g.member == g.p - 1 ? nx : s1 * kSegOut + 2;
+    }
+    // The predictor also takes the two cells beyond each shared edge of the
+    // range (the segment before and after, results kept to [w_lo, w_hi)), so
+    // that the corrector needs no other member's predictor results: the
+    // members meet once per step, and exchange the four end-of-step cells on
+    // either side of each shared edge that those extra cells read.  The extra
+    // cells' predictor results stay in LDS (ext[side][plane][cell]): the
+    // neighbour writes the same cells' results to the rows, and two workgroups
+    // writing one line through different caches could leave either's older
+    // bytes in memory.
+    const bool left = GANG && g.member > 0, right = GANG && g.member < g.p - 1;
+    const int sp0 = s0 - (left ? 1 : 0), sp1 = s1 + (right ? 1 : 0);
+    const int w_lo = c_lo - (left ? 2 : 0), w_hi = c_hi + (right ? 2 : 0);
+    // a cell another member writes, or one of ours another member reads
+    auto foreign = [&](int c) { return GANG && (c < c_lo || c >= c_hi); };
+    auto shared_out = [&](int c) { return (left && c < c_lo + 4) || (right && c >= c_hi - 4); };
+    auto ld = [&](const double *base, int c) -> double {
+        if (GANG && (foreign(c) || shared_out(c))) return load_c(&PL(base, c));
+        return PL(base, c);
+    };
+    auto st = [&](double *base, int c, double v) {
+        if (GANG && shared_out(c)) store_c(&PL(base, c), v);
+        else PL(base, c) = v;
+    };
+    const double *icr = a.ic + t.off;
+    double *str = a.st + t.off;
+    double *ssr = a.ss + t.off;
+    double *oldr = a.old + t.off;
+
+    StageArgs sa{};
+    sa.ds[0] = t.dx;
+    sa.inv_ds[0] = t.inv_dx;
+    sa.ds_pow2[0] = t.dx_pow2;
+    sa.gam = t.gam;
+    unsigned flg = 0;
+
+    // Courant maximum of the set-up (main.f90:232-241): every row cell
+    // with its own values, and the frame tuple at every c
+    double r0 = 0.0;
+    for (int c = c_lo + (int)threadIdx.x; c < c_hi; c += 64 * W) {
+        const double u0 = PL(icr + 5 * T, c), u1 = PL(icr + 6 * T, c), u2 = PL(icr + 7 * T, c), pi = PL(icr + 8 * T, c);
+        unsigned fset = 0;
+        if (c > 0 && c < nx - 1) {
+            const double rho = PL(icr, c);
+            double E = PL(icr + 4 * T, c);
+            const double p = eos_state(t, rho, E, u0, u1, u2, pi, fset);
+            r0 = vmax(r0, rate_of(t, rho, p, u0, u1, u2));
+        }
+        double Ef = t.seps;
+        const double pf = eos_state(t, t.srho, Ef, u0, u1, u2, pi, fset);
+        r0 = vmax(r0, rate_of(t, t.srho, pf, u0, u1, u2));
+    }
+    double rate = uniform_d(block_max(r0, red, W));
+    if (GANG && !gang_max(g, rate, s_ok, s_r)) return;
+
+    double time = 0.0, dtold = 0.0;
+    int step = 0;
+    unsigned status = 0;
+    for (;;) {
+        if (time >= t.endtime || step >= t.maxsteps) break;  // main.f90:212
+        step++;
+        // main.f90:266-281; every thread forms the same dt
+        double dt = t.cfl / rate;
+        dt = dt * 1.0;
+        if (step > 1 && dtold > 0.0) dt = vmin(dt, dtold * 1.1);
+        dtold = dt;
+        if (dt + time > t.endtime) dt = t.endtime - time;
+        dt = dt * 0.5;
+        dt = dt * 2.0;
+        if (!(dt > 0.0) || !__builtin_isfinite(dt)) {
+            status |= FLAG_BADDT;
+            step--;
+            break;
+        }
+        if (a.dtlog && threadIdx.x == 0 && (!GANG || g.member == 0) && step <= a.maxlog)
+            a.dtlog[(int64_t)ti * a.maxlog + step - 1] = dt;
+        sa.dt = dt;
+        const double *src = step == 1 ? icr : str;
+
+        // ---------------- predictor: main.f90:232-355
+        for (int s = sp0 + wv; s < sp1; s += W) {
+            const int c = s * kSegOut + lane;  // lane 2 is the segment's first output
+            const int d = donor_of(min(c, nx - 1), nx);
+            const bool ghost = c < 2 || c > nx - 3;
+            double U[5], v[3];
+            for (int k = 0; k < 5; k++) U[k] = ld(src + k * T, d);
+            const double w0 = ld(src + 5 * T, d), w1 = ld(src + 6 * T, d), w2 = ld(src + 7 * T, d);
+            const double p0 = ld(src + 8 * T, d);
+            // in the segments past an edge only the cells the kept outputs read
+            // count (the rest may hold an older state)
+            const bool count = !GANG || (s >= s0 && s < s1) || (c >= c_lo - 4 && c < c_hi + 4);
+            unsigned pf = 0;
+            const double p232 = eos_state(t, U[0], U[4], w0, w1, w2, p0, pf);    // :232
+            U[4] = source_E(t, U);                                               // :289
+            const double p314 = eos_state(t, U[0], U[4], w0, w1, w2, p232, pf);  // :314
+            velocity_of(t, U, v, pf);                                            // :318
+            if (count) flg |= pf;
+            W6<double> q;
+            q.d = U[0];
+            q.u0 = v[0];
+            q.u1 = v[1];
+            q.u2 = v[2];
+            q.p = p314;
+            q.e = floor_at(U[4] - 0.5 * U[0] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), t.seps);
+            double Fx[5];
+            x_faces(sa, q, sound_of(q, t.gam), ghost, Fx);
+            double dF[5];
+            for (int k = 0; k < 5; k++) dF[k] = lane_above(Fx[k]) - Fx[k];
+            __asm__ volatile("" ::: "memory");  // keep the update's loads after the faces (register pressure)
+            if (lane >= 2 && lane < 62 && !ghost && c >= w_lo && c < w_hi) {
+                double Us[5], us[3];
+                const bool mine = !GANG || (c >= c_lo && c < c_hi);
+                for (int k = 0; k < 5; k++) {
+                    if (mine) PL(oldr + k * T, c) = U[k];         // cons_old, main.f90:327-333
+                    Us[k] = U[k] + flux_tmp(t, dF[k], dt);      // flux.f90:205
+                }
+                Us[0] = floor_at(Us[0], t.srho);                // flux.f90:225-228
+                if (Us[4] < 0.0) Us[4] = 0.5 * (Us[1] * Us[1] + Us[2] * Us[2] + Us[3] * Us[3]) / Us[0] + t.seps;
+                const double ps = eos_state(t, Us[0], Us[4], v[0], v[1], v[2], p314, flg);  // :353
+                velocity_of(t, Us, us, flg);                                             // :355
+                if (mine) {
+                    for (int k = 0; k < 5; k++) PL(ssr + k * T, c) = Us[k];
+                    for (int k = 0; k < 3; k++) PL(ssr + (5 + k) * T, c) = us[k];
+                    PL(ssr + 8 * T, c) = ps;
+                } else {
+                    const int side = c < c_lo ? 0 : 1, j = c < c_lo ? c - w_lo : c - c_hi;
+                    for (int k = 0; k < 5; k++) ext[side][k][j] = Us[k];
+                    for (int k = 0; k < 3; k++) ext[side][5 + k][j] = us[k];
+                    ext[side][8][j] = ps;
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---------------- corrector: main.f90:368-515, then :232-241 of the next step
+        double rn = 0.0;
+        for (int s = s0 + wv; s < s1; s += W) {
+            const int c = s * kSegOut + lane;
+            const int d = donor_of(min(c, nx - 1), nx);
+            const bool ghost = c < 2 || c > nx - 3;
+            W6<double> q;
+            // the predictor results: the rows, or LDS past a shared edge
+            const bool xe = GANG && (d < c_lo || d >= c_hi);
+            const int xs = d < c_lo ? 0 : 1, xj = min(max(d < c_lo ? d - w_lo : d - c_hi, 0), 1);
+            auto sv = [&](int k) { return xe ? ext[xs][k][xj] : PL(ssr + k * T, d); };
+            q.d = sv(0);
+            const double E = sv(4);
+            q.u0 = sv(5);
+            q.u1 = sv(6);
+            q.u2 = sv(7);
+            q.p = sv(8);
+            q.e = floor_at(E - 0.5 * q.d * (q.u0 * q.u0 + q.u1 * q.u1 + q.u2 * q.u2), t.seps);
+            double Fx[5];
+            x_faces(sa, q, sound_of(q, t.gam), ghost, Fx);
+            double dF[5];
+            for (int k = 0; k < 5; k++) dF[k] = lane_above(Fx[k]) - Fx[k];
+            __asm__ volatile("" ::: "memory");  // keep the update's loads after the faces (register pressure)
+            if (lane >= 2 && lane < 62 && !ghost) {
+                double U[5], v[3], v2[3];
+                for (int k = 0; k < 5; k++)
+                    U[k] = 0.5 * (PL(oldr + k * T, c) + PL(ssr + k * T, c) + flux_tmp(t, dF[k], dt));  // flux.f90:213
+                U[0] = floor_at(U[0], t.srho);
+                if (U[4] < 0.0) U[4] = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0] + t.seps;
+                const double p376 = eos_state(t, U[0], U[4], q.u0, q.u1, q.u2, q.p, flg);    // :376
+                velocity_of(t, U, v, flg);                                                  // :378
+                U[4] = source_E(t, U);                                                      // :490
+                const double p513 = eos_state(t, U[0], U[4], v[0], v[1], v[2], p376, flg);  // :513
+                velocity_of(t, U, v2, flg);                                                 // :515
+                for (int k = 0; k < 5; k++) st(str + k * T, c, U[k]);
+                for (int k = 0; k < 3; k++) st(str + (5 + k) * T, c, v2[k]);
+                st(str + 8 * T, c, p513);
+                // the next step's state (:232) and Courant rate (:241)
+                double E2 = U[4];
+                unsigned fn = 0;
+                const double p2 = eos_state(t, U[0], E2, v2[0], v2[1], v2[2], p513, fn);
+                rn = vmax(rn, rate_of(t, U[0], p2, v2[0], v2[1], v2[2]));
+            }
+        }
+        rate = uniform_d(block_max(rn, red, W));  // its barriers also order this phase's stores before the next
+        if (GANG && !gang_max(g, rate, s_ok, s_r)) return;
+        time = time + dt;              // main.f90:545
+    }
+    if (step == 0) {
+        // no step taken: the state is the set-up
+        for (int c = c_lo + (int)threadIdx.x; c < c_hi; c += 64 * W)
+            for (int k = 0; k < 9; k++) st(str + k * T, c, PL(icr + k * T, c));
+    }
+    if (flg) atomicOr(sflag, flg);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (!GANG || g.member == 0) {
+            a.t_out[ti] = time;
+            a.steps_out[ti] = step;
+        }
+        atomicOr(&a.status[ti], *sflag | status);
+    }
+}
+
 template <int W>
 __global__ __launch_bounds__(64 * W) void k_tubes(BatchArgs a, const Tube *__restrict__ tubes, const int *__restrict__ order)
 {
     __shared__ double red[W];
     __shared__ int cur;
     __shared__ unsigned sflag;
-    const int lane = threadIdx.x & 63, wv = wave_id();  // wave-uniform for the compiler
-    const int64_t T = a.total;
+    __shared__ int s_ok;
+    __shared__ double s_r;
+    __shared__ double ext[2][9][2];
+    // the gangs first: workgroups [0, ngang * gang_p), gang blockIdx / gang_p
+    if (a.gang_p > 1 && (int)blockIdx.x < a.ngang * a.gang_p) {
+        if (threadIdx.x == 0) sflag = 0;
+        __syncthreads();
+        GangCtx g;
+        const int gi = (int)blockIdx.x / a.gang_p;
+        g.ctr = a.gang_ctr + gi;
+        g.rate = a.gang_rate + 3 * gi;
+        g.fail = a.gang_fail;
+        g.member = (int)blockIdx.x - gi * a.gang_p;
+        g.p = a.gang_p;
+        g.epoch = 0;
+        g.exch = 0;
+        const int ti = __builtin_amdgcn_readfirstlane(a.gang_tube[gi]);
+        const Tube t = tubes[ti];
+        run_tube<W, true>(a, t, ti, g, red, &s_ok, &s_r, &sflag, ext);
+    }
+    GangCtx none{};
     for (;;) {
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -219,165 +541,28 @@ __global__ __launch_bounds__(64 * W) void k_tubes(BatchArgs a, const Tube *__res
         }
         __syncthreads();
         const int qi = __builtin_amdgcn_readfirstlane(cur);
-        if (qi >= a.ntube) break;  // the whole workgroup leaves together
+        if (qi >= a.nqueue) break;  // the whole workgroup leaves together
         const int ti = __builtin_amdgcn_readfirstlane(order[qi]);
         const Tube t = tubes[ti];
-        const int nx = t.nx;
-        const int nseg = (nx - 4 + kSegOut - 1) / kSegOut;
-        const double *icr = a.ic + t.off;
-        double *str = a.st + t.off;
-        double *ssr = a.ss + t.off;
-        double *oldr = a.old + t.off;
-
-        StageArgs sa{};
-        sa.ds[0] = t.dx;
-        sa.inv_ds[0] = t.inv_dx;
-        sa.ds_pow2[0] = t.dx_pow2;
-        sa.gam = t.gam;
-        unsigned flg = 0;
-
-        // Courant maximum of the set-up (main.f90:232-241): every row cell
-        // with its own values, and the frame tuple at every c
-        double r0 = 0.0;
-        for (int c = threadIdx.x; c < nx; c += 64 * W) {
-            const double u0 = PL(icr + 5 * T, c), u1 = PL(icr + 6 * T, c), u2 = PL(icr + 7 * T, c), pi = PL(icr + 8 * T, c);
-            unsigned fset = 0;
-            if (c > 0 && c < nx - 1) {
-                const double rho = PL(icr, c);
-                double E = PL(icr + 4 * T, c);
-                const double p = eos_state(t, rho, E, u0, u1, u2, pi, fset);
-                r0 = vmax(r0, rate_of(t, rho, p, u0, u1, u2));
-            }
-            double Ef = t.seps;
-            const double pf = eos_state(t, t.srho, Ef, u0, u1, u2, pi, fset);
-            r0 = vmax(r0, rate_of(t, t.srho, pf, u0, u1, u2));
-        }
-        double rate = uniform_d(block_max(r0, red, W));
-
-        double time = 0.0, dtold = 0.0;
-        int step = 0;
-        unsigned status = 0;
-        for (;;) {
-            if (time >= t.endtime || step >= t.maxsteps) break;  // main.f90:212
-            step++;
-            // main.f90:266-281; every thread forms the same dt
-            double dt = t.cfl / rate;
-            dt = dt * 1.0;
-            if (step > 1 && dtold > 0.0) dt = vmin(dt, dtold * 1.1);
-            dtold = dt;
-            if (dt + time > t.endtime) dt = t.endtime - time;
-            dt = dt * 0.5;
-            dt = dt * 2.0;
-            if (!(dt > 0.0) || !__builtin_isfinite(dt)) {
-                status |= FLAG_BADDT;
-                step--;
-                break;
-            }
-            if (a.dtlog && threadIdx.x == 0 && step <= a.maxlog) a.dtlog[(int64_t)ti * a.maxlog + step - 1] = dt;
-            sa.dt = dt;
-            const double *src = step == 1 ? icr : str;
-
-            // ---------------- predictor: main.f90:232-355
-            for (int s = wv; s < nseg; s += W) {
-                const int c = s * kSegOut + lane;  // lane 2 is the segment's first output
-                const int d = donor_of(min(c, nx - 1), nx);
-                const bool ghost = c < 2 || c > nx - 3;
-                double U[5], v[3];
-                for (int k = 0; k < 5; k++) U[k] = PL(src + k * T, d);
-                const double w0 = PL(src + 5 * T, d), w1 = PL(src + 6 * T, d), w2 = PL(src + 7 * T, d);
-                const double p0 = PL(src + 8 * T, d);
-                const double p232 = eos_state(t, U[0], U[4], w0, w1, w2, p0, flg);    // :232
-                U[4] = source_E(t, U);                                                // :289
-                const double p314 = eos_state(t, U[0], U[4], w0, w1, w2, p232, flg);  // :314
-                velocity_of(t, U, v, flg);                                            // :318
-                W6<double> q;
-                q.d = U[0];
-                q.u0 = v[0];
-                q.u1 = v[1];
-                q.u2 = v[2];
-                q.p = p314;
-                q.e = floor_at(U[4] - 0.5 * U[0] * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]), t.seps);
-                double Fx[5];
-                x_faces(sa, q, sound_of(q, t.gam), ghost, Fx);
-                double dF[5];
-                for (int k = 0; k < 5; k++) dF[k] = lane_above(Fx[k]) - Fx[k];
-                __asm__ volatile("" ::: "memory");  // keep the update's loads after the faces (register pressure)
-                if (lane >= 2 && lane < 62 && !ghost) {
-                    double Us[5], us[3];
-                    for (int k = 0; k < 5; k++) {
-                        PL(oldr + k * T, c) = U[k];                     // cons_old, main.f90:327-333
-                        Us[k] = U[k] + flux_tmp(t, dF[k], dt);      // flux.f90:205
-                    }
-                    Us[0] = floor_at(Us[0], t.srho);                // flux.f90:225-228
-                    if (Us[4] < 0.0) Us[4] = 0.5 * (Us[1] * Us[1] + Us[2] * Us[2] + Us[3] * Us[3]) / Us[0] + t.seps;
-                    const double ps = eos_state(t, Us[0], Us[4], v[0], v[1], v[2], p314, flg);  // :353
-                    velocity_of(t, Us, us, flg);                                             // :355
-                    for (int k = 0; k < 5; k++) PL(ssr + k * T, c) = Us[k];
-                    for (int k = 0; k < 3; k++) PL(ssr + (5 + k) * T, c) = us[k];
-                    PL(ssr + 8 * T, c) = ps;
-                }
-            }
-            __syncthreads();
-
-            // ---------------- corrector: main.f90:368-515, then :232-241 of the next step
-            double rn = 0.0;
-            for (int s = wv; s < nseg; s += W) {
-                const int c = s * kSegOut + lane;
-                const int d = donor_of(min(c, nx - 1), nx);
-                const bool ghost = c < 2 || c > nx - 3;
-                W6<double> q;
-                q.d = PL(ssr, d);
-                const double E = PL(ssr + 4 * T, d);
-                q.u0 = PL(ssr + 5 * T, d);
-                q.u1 = PL(ssr + 6 * T, d);
-                q.u2 = PL(ssr + 7 * T, d);
-                q.p = PL(ssr + 8 * T, d);
-                q.e = floor_at(E - 0.5 * q.d * (q.u0 * q.u0 + q.u1 * q.u1 + q.u2 * q.u2), t.seps);
-                double Fx[5];
-                x_faces(sa, q, sound_of(q, t.gam), ghost, Fx);
-                double dF[5];
-                for (int k = 0; k < 5; k++) dF[k] = lane_above(Fx[k]) - Fx[k];
-                __asm__ volatile("" ::: "memory");  // keep the update's loads after the faces (register pressure)
-                if (lane >= 2 && lane < 62 && !ghost) {
-                    double U[5], v[3], v2[3];
-                    for (int k = 0; k < 5; k++)
-                        U[k] = 0.5 * (PL(oldr + k * T, c) + PL(ssr + k * T, c) + flux_tmp(t, dF[k], dt));  // flux.f90:213
-                    U[0] = floor_at(U[0], t.srho);
-                    if (U[4] < 0.0) U[4] = 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0] + t.seps;
-                    const double p376 = eos_state(t, U[0], U[4], q.u0, q.u1, q.u2, q.p, flg);    // :376
-                    velocity_of(t, U, v, flg);                                                  // :378
-                    U[4] = source_E(t, U);                                                      // :490
-                    const double p513 = eos_state(t, U[0], U[4], v[0], v[1], v[2], p376, flg);  // :513
-                    velocity_of(t, U, v2, flg);                                                 // :515
-                    for (int k = 0; k < 5; k++) PL(str + k * T, c) = U[k];
-                    for (int k = 0; k < 3; k++) PL(str + (5 + k) * T, c) = v2[k];
-                    PL(str + 8 * T, c) = p513;
-                    // the next step's state (:232) and Courant rate (:241)
-                    double E2 = U[4];
-                    unsigned fn = 0;
-                    const double p2 = eos_state(t, U[0], E2, v2[0], v2[1], v2[2], p513, fn);
-                    rn = vmax(rn, rate_of(t, U[0], p2, v2[0], v2[1], v2[2]));
-                }
-            }
-            rate = uniform_d(block_max(rn, red, W));  // its barriers also order this phase's stores before the next
-            time = time + dt;              // main.f90:545
-        }
-        if (step == 0) {
-            // no step taken: the state is the set-up
-            for (int c = threadIdx.x; c < nx; c += 64 * W)
-                for (int k = 0; k < 9; k++) PL(str + k * T, c) = PL(icr + k * T, c);
-        }
-        if (flg) atomicOr(&sflag, flg);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            a.t_out[ti] = time;
-            a.steps_out[ti] = step;
-            a.status[ti] = sflag | status;
-        }
+        run_tube<W, false>(a, t, ti, none, red, &s_ok, &s_r, &sflag, ext);
     }
 }
 
-__global__ void k_tubes_reset(int *queue) { *queue = 0; }
+// Clears the queue counter and the gangs' counters, maxima and failure word
+// (a kernel, not memsets, so that the launch can be captured).
+__global__ void k_tubes_reset(int *queue, int ngang, unsigned *ctr, unsigned long long *rate, unsigned *fail,
+                              unsigned *status, int ntube)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        *queue = 0;
+        if (fail) *fail = 0u;
+    }
+    if (i < ngang) ctr[i] = 0u;
+    if (i < 3 * ngang) rate[i] = 0ull;
+    if (i < ntube) status[i] = 0u;
+}
+
 
 }  // namespace bx
 
@@ -399,6 +584,16 @@ struct bx_batch {
     double *dtlog = nullptr;
     int maxlog = 0;
     int waves = 16, blocks_per_cu = 1, ncu = 0;
+    // gangs: the ngang costliest tubes, gang_p workgroups each (gang_p < 2: off)
+    int gang_p = 0, ngang = 0;
+    std::vector<int> hsteps;  // steps of each tube in the last finished run (empty: none yet)
+    int *gang_tube = nullptr;
+    unsigned *gang_ctr = nullptr;
+    unsigned long long *gang_rate = nullptr;
+    unsigned *gang_fail = nullptr;
+    int *qorder = nullptr;  // the queue order without the gang tubes
+    int nqueue = 0;
+    int gang_fallbacks = 0;
     int table_maxsteps = -1;  // maxsteps held by the device tube table
     bool graph_valid = false;
     hipGraph_t graph = nullptr;
@@ -478,8 +673,9 @@ int ensure_table(bx_batch *b, int maxsteps)
     return 0;
 }
 
-int enqueue(bx_batch *b)
+int enqueue(bx_batch *b, bool gangs = true)
 {
+    const bool g = gangs && b->gang_p > 1 && b->ngang > 0;
     bx::BatchArgs a;
     a.ntube = b->ntube;
     a.maxlog = b->maxlog;
@@ -495,18 +691,29 @@ int enqueue(bx_batch *b)
     a.steps_out = b->steps_out;
     a.status = b->status;
     a.dtlog = b->dtlog;
-    const unsigned grid = (unsigned)std::max(1, std::min(b->ntube, b->ncu * b->blocks_per_cu));
-    // the queue counter is reset by a kernel (not a memset), so the pair can be captured
-    hipLaunchKernelGGL(bx::k_tubes_reset, dim3(1), dim3(1), 0, b->stream, b->queue);
+    a.nqueue = g ? b->nqueue : b->ntube;
+    a.ngang = g ? b->ngang : 0;
+    a.gang_p = g ? b->gang_p : 0;
+    a.gang_tube = b->gang_tube;
+    a.gang_ctr = b->gang_ctr;
+    a.gang_rate = b->gang_rate;
+    a.gang_fail = b->gang_fail;
+    const int *order = g ? b->qorder : b->order;
+    const int gwg = a.ngang * a.gang_p;
+    const unsigned grid = (unsigned)std::max(std::max(1, gwg), std::min(a.nqueue + gwg, b->ncu * b->blocks_per_cu));
+    // the counters are reset by a kernel (not memsets), so the pair can be captured
+    const int nreset = std::max(std::max(b->ntube, 3 * a.ngang), 1);
+    hipLaunchKernelGGL(bx::k_tubes_reset, dim3((nreset + 255) / 256), dim3(256), 0, b->stream, b->queue, a.ngang,
+                       b->gang_ctr, b->gang_rate, b->gang_fail, b->status, b->ntube);
     switch (b->waves) {
     case 4:
-        hipLaunchKernelGGL(bx::k_tubes<4>, dim3(grid), dim3(256), 0, b->stream, a, b->tubes, b->order);
+        hipLaunchKernelGGL(bx::k_tubes<4>, dim3(grid), dim3(256), 0, b->stream, a, b->tubes, order);
         break;
     case 8:
-        hipLaunchKernelGGL(bx::k_tubes<8>, dim3(grid), dim3(512), 0, b->stream, a, b->tubes, b->order);
+        hipLaunchKernelGGL(bx::k_tubes<8>, dim3(grid), dim3(512), 0, b->stream, a, b->tubes, order);
         break;
     default:
-        hipLaunchKernelGGL(bx::k_tubes<16>, dim3(grid), dim3(1024), 0, b->stream, a, b->tubes, b->order);
+        hipLaunchKernelGGL(bx::k_tubes<16>, dim3(grid), dim3(1024), 0, b->stream, a, b->tubes, order);
         break;
     }
     BX_HIP(hipGetLastError());
@@ -547,7 +754,7 @@ void batch_free(bx_batch *b)
     for (auto &e : b->ev)
         if (e) (void)hipEventDestroy(e);
     void *ptrs[] = {b->tubes, b->order, b->queue, b->ic, b->st, b->ss, b->old, b->t_out, b->steps_out, b->status,
-                    b->dtlog};
+                    b->dtlog, b->gang_tube, b->gang_ctr, b->gang_rate, b->gang_fail, b->qorder};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -751,11 +958,80 @@ int bx_batch_sync(bx_batch *b, double *ms)
     if (!b) return bx::set_error(BX_ERR_ARG, "bx_batch_sync: null batch");
     BX_HIP(hipSetDevice(b->device));
     BX_HIP(hipStreamSynchronize(b->stream));
+    if (b->gang_p > 1 && b->ngang > 0 && b->timed) {
+        unsigned fail = 0;
+        BX_HIP(hipMemcpy(&fail, b->gang_fail, sizeof(unsigned), hipMemcpyDeviceToHost));
+        if (fail) {
+            // a gang's members were not resident together: the batch again, without gangs
+            b->gang_fallbacks++;
+            BX_HIP(hipEventRecord(b->ev[0], b->stream));
+            if (int rc = enqueue(b, false)) return rc;
+            BX_HIP(hipEventRecord(b->ev[1], b->stream));
+            BX_HIP(hipStreamSynchronize(b->stream));
+        }
+    }
+    if (b->timed) {  // the steps of each tube, the measured costs for bx_batch_gang
+        b->hsteps.resize(b->ntube);
+        BX_HIP(hipMemcpy(b->hsteps.data(), b->steps_out, sizeof(int) * b->ntube, hipMemcpyDeviceToHost));
+    }
     if (ms) {
         float f = 0.0f;
         if (b->timed) BX_HIP(hipEventElapsedTime(&f, b->ev[0], b->ev[1]));
         *ms = f;
     }
+    return 0;
+}
+
+int bx_batch_gang(bx_batch *b, int32_t members, int32_t ntubes)
+{
+    if (!b) return bx::set_error(BX_ERR_ARG, "bx_batch_gang: null batch");
+    if (members < 0 || ntubes < 0 || members > 64)
+        return bx::set_error(BX_ERR_ARG, "bx_batch_gang: members in [0,64] and ntubes >= 0");
+    BX_HIP(hipSetDevice(b->device));
+    BX_HIP(hipStreamSynchronize(b->stream));
+    b->graph_valid = false;
+    if (members < 2 || ntubes == 0) {
+        b->gang_p = 0;
+        b->ngang = 0;
+        return 0;
+    }
+    if (ntubes > b->ntube) return bx::set_error(BX_ERR_ARG, "bx_batch_gang: more gangs than tubes");
+    if ((int64_t)members * ntubes > b->ncu)
+        return bx::set_error(BX_ERR_ARG, "bx_batch_gang: members x ntubes must not exceed the CUs "
+                                         "(the members of a gang run together)");
+    // the costliest tubes: measured (steps x interior cells of the last run) when a
+    // run has finished, else the set-up estimate's order
+    std::vector<int> ord(b->horder);
+    if ((int)b->hsteps.size() == b->ntube) {
+        std::vector<double> cost(b->ntube);
+        for (int i = 0; i < b->ntube; i++) cost[i] = (double)b->hsteps[i] * (double)(b->htubes[i].nx - 4);
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    }
+    std::vector<int> gt(ord.begin(), ord.begin() + ntubes), q(ord.begin() + ntubes, ord.end());
+    for (void *p : {(void *)b->gang_tube, (void *)b->gang_ctr, (void *)b->gang_rate, (void *)b->gang_fail,
+                    (void *)b->qorder})
+        if (p) (void)hipFree(p);
+    b->gang_tube = b->qorder = nullptr;
+    b->gang_ctr = b->gang_fail = nullptr;
+    b->gang_rate = nullptr;
+    b->gang_p = b->ngang = 0;
+    BX_HIP(hipMalloc((void **)&b->gang_tube, sizeof(int) * ntubes));
+    BX_HIP(hipMalloc((void **)&b->gang_ctr, sizeof(unsigned) * ntubes));
+    BX_HIP(hipMalloc((void **)&b->gang_rate, sizeof(unsigned long long) * 3 * ntubes));
+    BX_HIP(hipMalloc((void **)&b->gang_fail, sizeof(unsigned)));
+    BX_HIP(hipMalloc((void **)&b->qorder, sizeof(int) * std::max<size_t>(q.size(), 1)));
+    BX_HIP(hipMemcpy(b->gang_tube, gt.data(), sizeof(int) * ntubes, hipMemcpyHostToDevice));
+    if (!q.empty()) BX_HIP(hipMemcpy(b->qorder, q.data(), sizeof(int) * q.size(), hipMemcpyHostToDevice));
+    b->gang_p = members;
+    b->ngang = ntubes;
+    b->nqueue = (int)q.size();
+    return 0;
+}
+
+int bx_batch_gang_fallbacks(const bx_batch *b, int32_t *n)
+{
+    if (!b || !n) return bx::set_error(BX_ERR_ARG, "bx_batch_gang_fallbacks: null argument");
+    *n = b->gang_fallbacks;
     return 0;
 }
 

@@ -108,6 +108,8 @@ _SIGS = {
     "bx_batch_cells": [_VP, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)],
     "bx_batch_order": [_VP, ctypes.POINTER(_I32)],
     "bx_batch_tune": [_VP, _I32, _I32],
+    "bx_batch_gang": [_VP, _I32, _I32],
+    "bx_batch_gang_fallbacks": [_VP, ctypes.POINTER(_I32)],
     "bx_batch_log_dt": [_VP, _I32],
     "bx_batch_run": [_VP, _I32, _I32],
     "bx_batch_sync": [_VP, _DP],

@@ -219,8 +219,23 @@ int bx_batch_log_dt(bx_batch *b, int32_t maxlog);
  * captured hipGraph of the launch.  Asynchronous. */
 int bx_batch_run(bx_batch *b, int32_t maxsteps, int32_t use_graph);
 
-/* Wait for the run; *ms receives its duration on the device (HIP events). */
+/* Wait for the run; *ms receives its duration on the device (HIP events).
+ * When a gang could not run together (its members were not resident at the
+ * same time) the run is repeated without gangs before this returns, and *ms
+ * is that run's. */
 int bx_batch_sync(bx_batch *b, double *ms);
+
+/* Gangs: the ntubes costliest tubes (by the steps of the last finished run,
+ * else by the set-up estimate) are each split over `members` workgroups
+ * (contiguous ranges of cells; the two cells at either side of a range are
+ * exchanged through device-coherent accesses, with two device-counter
+ * barriers per step).  members x ntubes must not exceed the compute units.
+ * members < 2 or ntubes = 0 turns gangs off.  Results do not change.  No
+ * reference counterpart: the reference runs one tube per process. */
+int bx_batch_gang(bx_batch *b, int32_t members, int32_t ntubes);
+
+/* Runs repeated without gangs since the batch was created. */
+int bx_batch_gang_fallbacks(const bx_batch *b, int32_t *n);
 
 /* Final rows (cons(5,.), u(3,.), p(.); any may be NULL), time, steps and
  * status bits (1: the low-density EOS branch was met, 2: the vlimit clamp

@@ -265,11 +265,14 @@ def test_long_tube_goldens_match_generator():
 
 
 @pytest.mark.gpu
-def test_batch_long_tubes_match_reference_goldens():
+@pytest.mark.parametrize("gang", [(0, 0), (4, 32)])
+def test_batch_long_tubes_match_reference_goldens(gang):
     """Inside the whole 512-tube batch, the three tubes that set its run time
     (up to 4,049 cells and 16,891 steps) against the reference's runs of their
     full grids: step count, every dt (SHA-256 of the sequence, and every 64th
-    value), final time, and the final cons/u/p of the expanded rows."""
+    value), final time, and the final cons/u/p of the expanded rows.  With
+    gangs, the 32 costliest tubes (these three among them) each run over 4
+    workgroups."""
     import hashlib
 
     from boxzyhydro_amd.batch import expand_row
@@ -278,9 +281,14 @@ def test_batch_long_tubes_match_reference_goldens():
     allt = ics.riemann_tubes(512, seed=0)
     b = _batch(allt)
     b.log_dt(max(r["steps"] for r in recs.values()))
-    print("512 tubes, long-tube goldens", flush=True)
+    print(f"512 tubes, long-tube goldens, gangs {gang}", flush=True)
+    if gang[0]:
+        b.run(graph=True)
+        b.sync()  # measures every tube's steps, which pick the gang tubes
+        b.gang(*gang)
     b.run(graph=True)
     b.sync()
+    assert b.gang_fallbacks() == 0
     res = b.result()
     dts = b.dts()
     for name, rec in recs.items():
@@ -295,3 +303,45 @@ def test_batch_long_tubes_match_reference_goldens():
         full = expand_row(prm, r)
         for k in ("cons", "u", "p"):
             assert sha(full[k]) == rec["final_sha256"][k], (name, k)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("gang", [(2, 6), (3, 4), (4, 8)])
+def test_batch_gangs_match_one_workgroup_per_tube(gang):
+    """Gangs (each of the costliest tubes over several workgroups, the cells at
+    the shared edges exchanged through device-coherent accesses) give the bits
+    of one workgroup per tube: final rows, time, steps, status and every dt."""
+    tubes = ics.riemann_tubes(24, seed=5)
+    b = _batch(tubes)
+    b.log_dt(4000)
+    b.run(graph=False)
+    b.sync()
+    ref, ref_dt = b.result(), b.dts()
+    b.gang(*gang)
+    print(f"24 tubes, gangs {gang}", flush=True)
+    for graph in (False, True):
+        b.run(graph=graph)
+        b.sync()
+        assert b.gang_fallbacks() == 0
+        got, got_dt = b.result(), b.dts()
+        for i, (x, y) in enumerate(zip(got, ref)):
+            for k in ("cons", "u", "p"):
+                assert _same(x[k], y[k]), (gang, graph, i, k)
+            assert (x["time"], x["steps"], x["status"]) == (y["time"], y["steps"], y["status"]), (gang, i)
+            assert _same(got_dt[i], ref_dt[i]), (gang, i)
+
+
+@pytest.mark.gpu
+def test_batch_gang_limits():
+    """members x gangs must fit the compute units; members < 2 turns gangs off."""
+    from boxzyhydro_amd import native as nv
+
+    b = _batch(ics.riemann_tubes(8, seed=1))
+    with pytest.raises(nv.BoxzyError):
+        b.gang(64, 8)   # 512 workgroups: above the compute units
+    with pytest.raises(nv.BoxzyError):
+        b.gang(2, 9)    # more gangs than tubes
+    b.gang(1, 4)        # off
+    b.run(graph=False)
+    b.sync()
+    assert b.gang_fallbacks() == 0
