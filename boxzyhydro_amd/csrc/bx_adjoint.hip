@@ -593,7 +593,11 @@ __global__ __launch_bounds__(256) void k_adj_x(StageArgs a, const double *__rest
             for (int k = 0; k < 6; k++) elem<double>(LW, (int64_t)k * n + zb, coff) = lw[k];
         } else {
             // an interior cell that donates to no ghost: the rest of the stage
+            // (the bases laundered so that these loads are made here, not kept
+            // in registers from the kernel's first loads of the same cells)
             const double *U = (const double *)a.uin;
+            const double *lamr = lam;
+            __asm__ volatile("" : "+s"(U), "+s"(lamr));
             const double r = elem<double>(U, zb, coff), E = elem<double>(U, 4 * n + zb, coff);
             const double m[3] = {elem<double>(U, n + zb, coff), elem<double>(U, 2 * n + zb, coff),
                                  elem<double>(U, 3 * n + zb, coff)};
@@ -604,7 +608,7 @@ __global__ __launch_bounds__(256) void k_adj_x(StageArgs a, const double *__rest
                 const double m0[3] = {elem<double>(U0, n + zb, coff), elem<double>(U0, 2 * n + zb, coff),
                                       elem<double>(U0, 3 * n + zb, coff)};
                 double hl[5];  // lam/2, loaded again rather than held through the reverse
-                for (int f = 0; f < 5; f++) hl[f] = gscale * elem<double>(lam, (int64_t)f * n + zb, coff);
+                for (int f = 0; f < 5; f++) hl[f] = gscale * elem<double>(lamr, (int64_t)f * n + zb, coff);
                 mid2_tail(out, hl, elem<double>(U0, zb, coff), m0, r, E, a.gm1, a.seps, ls, lsum);
                 for (int f = 0; f < 5; f++) {
                     elem<double>(tail.lus, (int64_t)f * n + zb, coff) = ls[f];
