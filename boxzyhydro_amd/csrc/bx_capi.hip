@@ -535,6 +535,7 @@ int bx_upload(bx_ctx *c, const double *cons, const double *u, const double *p, c
     c->fused_valid = false;
     c->aux_stale = false;
     c->old_stale = false;
+    c->p_fused = false;
     if (cons) c->ghosts_filled = false;
     return BX_OK;
 }
@@ -753,6 +754,7 @@ static int exact_step(bx_ctx *c, double endtime, double *time, double *dtold, in
     *time = *time + dt;
     *dt_out = dt;
     c->old_stale = false;
+    c->p_fused = false;
     c->nexact++;
     return BX_OK;
 }
@@ -844,6 +846,7 @@ static int graph_chunk(bx_ctx *c, int K, double endtime, double *time, double *d
         for (int i = 0; i < K; i++) dts[i] = log[i];
     c->aux_stale = true;
     c->old_stale = true;
+    c->p_fused = c->grav_flags != 0;
     c->graph_steps += K;
     c->nfused += K;
     *taken = K;
@@ -868,12 +871,16 @@ static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, d
                 if ((rc = launch_materialize(c))) return rc;
                 c->aux_stale = false;
             }
-            bool ok = c->ghosts_filled;
+            // with a potential, gforce must be the force of the current phi
+            // (an exact step computes it), so that source(0) and source(1)
+            // of every fused step see the same force
+            bool ok = c->ghosts_filled && grav_settled(c);
             if (ok) {
                 if ((rc = launch_regular_check(c))) return rc;
                 if ((rc = read_scalars(c))) return rc;
                 ok = c->hsc->flags == 0;
             }
+            if (ok && (rc = grav_fused_planes(c))) return rc;
             if (!ok) {
                 double dt;
                 if ((rc = exact_step(c, endtime, time, dtold, step, &dt))) return rc;
@@ -942,6 +949,7 @@ static int fused_loop(bx_ctx *c, int32_t nsteps, double endtime, double *time, d
         else std::swap(c->cons, c->unext);
         c->nfused++;
         c->aux_stale = true;
+        c->p_fused = c->grav_flags != 0;
         c->old_stale = true;
         *time = *time + dt;  // main.f90:544-545 (dt*half*two == dt)
         if (dts) dts[k] = dt;

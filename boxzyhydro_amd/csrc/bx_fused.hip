@@ -41,7 +41,7 @@ namespace bx {
 // LDS, marching in z (bx_tile.h: tile_stage).  P2: every spacing and the
 // cell volume are powers of two (their divisions are multiplications), known
 // at compile time, which keeps the spacings themselves out of the registers.
-template <int STAGE, typename S, int TY, bool P2>
+template <int STAGE, typename S, int TY, bool P2, bool G = false>
 __global__ __launch_bounds__(64 * (TY + 4)) void k_stage(StageArgs a0)
 {
     __shared__ TileLds<S, TY> lds;
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(64 * (TY + 4)) void k_stage(StageArgs a0)
     const unsigned L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
     const unsigned xcd = L & 7u, slot = L >> 3, q = T >> 3, rem = T & 7u;
     const unsigned t = xcd * q + (xcd < rem ? xcd : rem) + slot;
-    tile_stage<STAGE, S, TY>(a, lds, t % gx, (t / gx) % gy, t / (gx * gy));
+    tile_stage<STAGE, S, TY, G>(a, lds, t % gx, (t / gx) % gy, t / (gx * gy));
 }
 
 
@@ -80,6 +80,7 @@ __global__ __launch_bounds__(256, OCC) void k_xz(StageArgs a0, const S *__restri
 }
 
 // -------------------------------------------------------------- helpers
+// p: null keeps the pressure plane (written by the fused corrector with gravity)
 __global__ __launch_bounds__(256) void k_materialize(int64_t n, const double *__restrict__ cons,
                                                     double *__restrict__ u, double *__restrict__ p,
                                                     double *__restrict__ ad, double gam, double seps)
@@ -165,6 +166,7 @@ StageArgs base_args(bx_ctx *c, double dt)
     a.nd2 = c->nd2;
     a.d2cell = c->d2_src;
     a.d2ubar = c->d2_ubar;
+    a.gf = c->grav_flags ? c->gfused : nullptr;
     return a;
 }
 
@@ -176,10 +178,20 @@ static int launch_tile(bx_ctx *c, const StageArgs &a)
     const unsigned gx = (unsigned)((c->nx - 4 + 63) / 64);
     const unsigned gy = (unsigned)((c->ny - 4 + TY - 1) / TY);
     const unsigned gz = (unsigned)((nzi + a.lz - 1) / a.lz);
+    const dim3 grid(gx, gy, gz), block(64 * (TY + 4));
+    if constexpr (sizeof(S) == 8 && TY == 8) {
+        if (a.gf) {
+            if (p2) hipLaunchKernelGGL((k_stage<STAGE, S, TY, true, true>), grid, block, 0, c->stream, a);
+            else hipLaunchKernelGGL((k_stage<STAGE, S, TY, false, true>), grid, block, 0, c->stream, a);
+            BX_HIP(hipGetLastError());
+            return BX_OK;
+        }
+    }
+    if (a.gf) return set_error(BX_ERR_UNSUPPORTED, "fused step with gravity: FP64 64 x 8 tiles only");
     if (p2)
-        hipLaunchKernelGGL((k_stage<STAGE, S, TY, true>), dim3(gx, gy, gz), dim3(64 * (TY + 4)), 0, c->stream, a);
+        hipLaunchKernelGGL((k_stage<STAGE, S, TY, true>), grid, block, 0, c->stream, a);
     else
-        hipLaunchKernelGGL((k_stage<STAGE, S, TY, false>), dim3(gx, gy, gz), dim3(64 * (TY + 4)), 0, c->stream, a);
+        hipLaunchKernelGGL((k_stage<STAGE, S, TY, false>), grid, block, 0, c->stream, a);
     BX_HIP(hipGetLastError());
     return BX_OK;
 }
@@ -187,6 +199,7 @@ static int launch_tile(bx_ctx *c, const StageArgs &a)
 template <int STAGE, typename S>
 static int launch_stage(bx_ctx *c, const StageArgs &a, hipEvent_t mid, S *eflux)
 {
+    if (c->grav_flags && !a.gf) return set_error(BX_ERR_STATE, "fused step with gravity before its force planes");
     if (c->fused_occ >= 10) {
         if (mid) BX_HIP(hipEventRecord(mid, c->stream));
         switch (c->fused_occ) {
@@ -194,6 +207,7 @@ static int launch_stage(bx_ctx *c, const StageArgs &a, hipEvent_t mid, S *eflux)
         default: return launch_tile<STAGE, S, 8>(c, a);
         }
     }
+    if (a.gf) return set_error(BX_ERR_UNSUPPORTED, "fused step with gravity: the tiled kernel only (variant 11)");
     const int nzi = a.zhi - a.zlo;
     const int ly = c->fused_ly;
     {
@@ -232,8 +246,8 @@ __global__ __launch_bounds__(64) void k_corner_pressure(int nd2, const int32_t *
 
 int launch_materialize(bx_ctx *c)
 {
-    hipLaunchKernelGGL(k_materialize, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cons, c->u, c->p,
-                       c->adindx, c->prm.gammafix, c->prm.small_eps);
+    hipLaunchKernelGGL(k_materialize, dim3(blocks_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->cons, c->u,
+                       c->p_fused ? (double *)nullptr : c->p, c->adindx, c->prm.gammafix, c->prm.small_eps);
     BX_HIP(hipGetLastError());
     if (c->nd2 > 0 && c->prm.precision == BX_PREC_FP64) {  // the mixed form copies the donors
         hipLaunchKernelGGL(k_corner_pressure, dim3(1), dim3(64), 0, c->stream, c->nd2, c->d2_dst, c->d2_ubar, c->p);
@@ -251,7 +265,11 @@ int fused_supported(const bx_ctx *c)
     if (c->grav[0] != 0.0 || c->grav[1] != 0.0 || c->grav[2] != 0.0)
         return set_error(BX_ERR_UNSUPPORTED, "fused step assumes zero gravity");
     if (c->prm.fluxangmom) return set_error(BX_ERR_UNSUPPORTED, "fused step fluxes linear momentum (fluxangmom off)");
-    if (c->grav_flags) return set_error(BX_ERR_UNSUPPORTED, "fused step assumes the zero-potential force");
+    if (c->grav_flags & BX_GRAV_EXTERNAL)
+        return set_error(BX_ERR_UNSUPPORTED, "fused step with gravity: a given potential only (no point mass, whose "
+                                             "force needs the corrector's mass moments before source(1))");
+    if (c->grav_flags && c->nd2 > 0)
+        return set_error(BX_ERR_UNSUPPORTED, "fused step with gravity: no two-level corner ghosts");
     if (c->eos) return set_error(BX_ERR_UNSUPPORTED, "fused step assumes a fixed gamma (H2STAT=-1)");
     return BX_OK;
 }
@@ -467,6 +485,21 @@ int widen_cons(bx_ctx *c)
     return BX_OK;
 }
 
+// With gravity the corrector also writes the pressure plane (the state after
+// source(1), which takes the velocity from before the kick); returns that
+// plane for the ghost fill, null without gravity.
+template <typename S>
+static S *grav_pressure(bx_ctx *c, StageArgs &b)
+{
+    if constexpr (sizeof(S) == 8) {
+        if (b.gf) {
+            b.pout = c->p;
+            return c->p;
+        }
+    }
+    return nullptr;
+}
+
 // One fused step from U^n (c->cons, or c->cons32 in the mixed form; regular)
 // with full dt; result in c->unext / c->unext32.  The Courant rate of U^{n+1}
 // accumulates into rate_bits[0] and irregular branches into flags (both
@@ -494,10 +527,11 @@ static int fused_step_t(bx_ctx *c, double dt, S *u0, S *us, S *ps, S *un, bool r
     b.uold = u0;
     b.uout = un;
     b.rate = &c->dsc->rate_bits[0];
+    S *pfin = grav_pressure<S>(c, b);
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[3], c->stream));
     if ((rc = launch_stage<2, S>(c, b, c->prof_on ? c->ev[4] : nullptr, (S *)c->eflux))) return rc;
     if (c->prof_on) BX_HIP(hipEventRecord(c->ev[5], c->stream));
-    if ((rc = ghosts6<S>(c, un, (S *)nullptr))) return rc;
+    if ((rc = ghosts6<S>(c, un, pfin))) return rc;
     if constexpr (sizeof(S) == 8) return corner_ghosts(c, b);
     return BX_OK;
 }
@@ -561,8 +595,9 @@ static int enqueue_pair(bx_ctx *c, S *u0, S *us, S *ps, S *un)
         b.uold = src;
         b.uout = dst;
         b.rate = &c->dsc->rate_bits[0];
+        S *pfin = grav_pressure<S>(c, b);
         if ((rc = launch_stage<2, S>(c, b, nullptr, (S *)c->eflux))) return rc;
-        if ((rc = ghosts6<S>(c, dst, (S *)nullptr))) return rc;
+        if ((rc = ghosts6<S>(c, dst, pfin))) return rc;
         if constexpr (sizeof(S) == 8)
             if ((rc = corner_ghosts(c, b))) return rc;
     }
@@ -649,6 +684,7 @@ __global__ void k_clear_scalars(unsigned int *flags, unsigned long long *rate)
 int fused_step_words(bx_ctx *c, double dt, const double *u0, double *us, double *ps, double *un, double *ey,
                      unsigned int *flags, unsigned long long *rate)
 {
+    if (c->grav_flags) return set_error(BX_ERR_UNSUPPORTED, "this fused step entry takes no gravity configuration");
     int rc;
     hipLaunchKernelGGL(k_clear_scalars, dim3(1), dim3(1), 0, c->stream, flags, rate);
     BX_HIP(hipGetLastError());

@@ -161,10 +161,47 @@ __global__ __launch_bounds__(kGThreads) void k_external_grav(int64_t n, int nx, 
 
 void gravity_free(bx_ctx *c)
 {
-    for (double *p : {c->phi, c->gforce, c->grav_sums})
+    for (double *p : {c->phi, c->gforce, c->grav_sums, c->gfused})
         if (p) (void)hipFree(p);
-    c->phi = c->gforce = c->grav_sums = nullptr;
+    c->phi = c->gforce = c->grav_sums = c->gfused = nullptr;
     c->grav_flags = 0;
+    c->gforce_gen = c->gfused_gen = -1;
+}
+
+// The ghosts of the force planes as the fused ghost fill sets the momenta's:
+// each ghost takes its (interior) donor's force with the ghost's sign flips.
+__global__ __launch_bounds__(kGThreads) void k_gf_ghosts(int64_t n, int64_t ng, const int32_t *__restrict__ dst,
+                                                        const int32_t *__restrict__ src,
+                                                        const int32_t *__restrict__ sign, double *__restrict__ gf)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ng) return;
+    const int64_t d = dst[k], s = src[k];
+    const int sg = sign[k];
+    for (int f = 0; f < 3; f++) {
+        const double v = gf[f * n + s];
+        gf[f * n + d] = ((sg >> f) & 1) ? -v : v;
+    }
+}
+
+bool grav_settled(const bx_ctx *c) { return !c->grav_flags || c->gforce_gen == c->phi_gen; }
+
+// The force planes of the fused kernels from the settled gforce (allocates
+// them on first use; a no-op while they are current).
+int grav_fused_planes(bx_ctx *c)
+{
+    if (!c->grav_flags || c->gfused_gen == c->gforce_gen) return BX_OK;
+    if (!c->gfused) BX_HIP(hipMalloc((void **)&c->gfused, sizeof(double) * 3 * (size_t)c->n));
+    BX_HIP(hipMemcpyAsync(c->gfused, c->gforce, sizeof(double) * 3 * (size_t)c->n, hipMemcpyDeviceToDevice,
+                          c->stream));
+    if (c->nghost > 0) {
+        hipLaunchKernelGGL(k_gf_ghosts, dim3(blocks_for(c->nghost, kGThreads)), dim3(kGThreads), 0, c->stream, c->n,
+                           c->nghost, c->ghost_dst, c->nscratch ? c->fghost_src : c->ghost_src,
+                           c->nscratch ? c->fghost_sign : c->ghost_sign, c->gfused);
+        BX_HIP(hipGetLastError());
+    }
+    c->gfused_gen = c->gforce_gen;
+    return BX_OK;
 }
 
 static int nblocks(const bx_ctx *c) { return (int)blocks_for(c->n, kGThreads); }
@@ -174,11 +211,15 @@ int launch_calc_gforce(bx_ctx *c)
     hipLaunchKernelGGL(k_calc_gforce, dim3(blocks_for(c->n, kGThreads)), dim3(kGThreads), 0, c->stream, c->n, c->nx,
                        c->ny, c->nz, c->flatz, c->phi, c->gforce, c->prm.dx, c->prm.dy, c->prm.dz);
     BX_HIP(hipGetLastError());
+    // the interior from phi alone; the ghosts (never read by source()) keep
+    // their values
+    c->gforce_gen = c->phi_gen;
     return BX_OK;
 }
 
 int launch_external_grav(bx_ctx *c)
 {
+    c->gforce_gen = -1;  // the point mass adds to gforce: no longer phi's alone
     const double dx = c->prm.dx, dy = c->prm.dy, dz = c->prm.dz;
     if (!(c->grav_flags & BX_GRAV_COM_SEQUENTIAL)) {
         const int nb = nblocks(c);
@@ -224,7 +265,11 @@ extern "C" int bx_gravity_config(bx_ctx *c, int32_t flags, double object_mass, d
     c->grav_flags = flags;
     c->obj_mass = object_mass;
     c->obj_radius = object_radius;
-    c->fused_unsupported = true;  // bx_step runs the exact sequence
+    c->phi_gen++;  // gforce (zero) is not yet the force of phi
+    c->fused_valid = false;
+    // fused_prepare decides on the next fused step: a potential alone takes
+    // the fused kernels, the point mass the exact sequence
+    c->fused_unsupported = false;
     return BX_OK;
 }
 
@@ -233,6 +278,8 @@ extern "C" int bx_set_phi(bx_ctx *c, const double *phi)
     if (!c) return set_error(BX_ERR_ARG, "bx_set_phi: null context");
     if (!c->grav_flags) return set_error(BX_ERR_STATE, "bx_set_phi: configure gravity first (bx_gravity_config)");
     BX_HIP(hipSetDevice(c->device));
+    c->phi_gen++;  // gforce follows at the next calc_gforce
+    c->fused_valid = false;
     if (!phi) {
         BX_HIP(hipMemsetAsync(c->phi, 0, sizeof(double) * c->n, c->stream));
         return BX_OK;

@@ -136,6 +136,16 @@ struct bx_ctx {
     double *phi;                   // n
     double *gforce;                // 3 planes
     double *grav_sums;             // centre-of-mass sums (3) and tree partials
+    // The fused step with a potential (BX_GRAV_POTENTIAL alone): gforce is
+    // settled when it was computed from the current phi (gforce_gen ==
+    // phi_gen), so that source(0) and source(1) of a step see the same force;
+    // gfused is gforce with the ghosts filled as their donors' (sign flips of
+    // the momenta), built from generation gfused_gen.  p_fused: the pressure
+    // plane holds what the last step's exact sequence would leave (written by
+    // the fused corrector), so materialising keeps it.
+    int64_t phi_gen, gforce_gen, gfused_gen;
+    double *gfused;  // 3 planes
+    bool p_fused;
 };
 
 namespace bx {
@@ -182,6 +192,8 @@ void eos_free(bx_ctx *c);
 void gravity_free(bx_ctx *c);
 int launch_calc_gforce(bx_ctx *c);
 int launch_external_grav(bx_ctx *c);
+bool grav_settled(const bx_ctx *c);
+int grav_fused_planes(bx_ctx *c);
 int launch_state_tab(bx_ctx *c);
 int fused_prepare(bx_ctx *c);
 // BX_STEP_GRAPH: replay `pairs` captured pairs of fused steps from the current

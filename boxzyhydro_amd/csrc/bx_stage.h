@@ -51,6 +51,11 @@ struct StageArgs {
     int nd2;
     const int32_t *d2cell;
     double *d2ubar;
+    // gravity (the kernels' G variant): 3 planes of the force that source()
+    // applies to each cell (source.f90:148-182), every ghost holding its
+    // donor's force with the ghost's momentum sign flips, so that a ghost's
+    // source result is its donor's up to those flips; null without gravity
+    const double *gf;
 };
 
 // The arguments of a launch with dt resolved: a kernel reads it once at entry.
@@ -208,10 +213,11 @@ __device__ __forceinline__ T sound_of(const W6<T> &w, double gam)
 template <typename S>
 struct Raw {
     S r, m0, m1, m2, E, p;
+    S g0, g1, g2;  // stage 1 with gravity: the cell's force
 };
 
 // cell = uniform element offset u + per-lane byte offset boff
-template <int STAGE, typename S>
+template <int STAGE, typename S, bool G = false>
 __device__ __forceinline__ Raw<S> load_raw(const StageArgs &a, int64_t u, unsigned boff)
 {
     const int64_t n = a.n;
@@ -222,18 +228,66 @@ __device__ __forceinline__ Raw<S> load_raw(const StageArgs &a, int64_t u, unsign
     v.m2 = elem<S>(a.uin, 3 * n + u, boff);
     v.E = elem<S>(a.uin, 4 * n + u, boff);
     v.p = STAGE == 2 ? elem<S>(a.pin, u, boff) : S(0);
+    if constexpr (G && STAGE == 1) {
+        v.g0 = elem<S>(a.gf, u, boff);
+        v.g1 = elem<S>(a.gf, n + u, boff);
+        v.g2 = elem<S>(a.gf, 2 * n + u, boff);
+    }
     return v;
+}
+
+// source.f90:148-182 at a cell (rho r, momenta m, energy E) with force g over
+// the half step dth: the momenta m + (rho g) dth, then the energy re-round
+// with the kinetic energy ek of the new momenta and the velocity u of the last
+// velocity call (`vmax(E - ekin + dt*rho*(u.g), eps) + ekin`).  The caller
+// forms ek (a quotient by rho it may share a reciprocal with).
+__device__ __forceinline__ void grav_momenta(double r, const double m[3], const double g[3], double dth, double mk[3])
+{
+    mk[1] = m[1] + (r * g[1]) * dth;
+    mk[0] = m[0] + (r * g[0]) * dth;
+    mk[2] = m[2] + (g[2] * r) * dth;
+}
+__device__ __forceinline__ double grav_energy_arg(double r, double E, double ek, const double u[3], const double g[3],
+                                                  double dth)
+{
+    return (E - ek) + (dth * r) * ((u[0] * g[0] + u[1] * g[1]) + u[2] * g[2]);
 }
 
 // primitive variables of a loaded cell for each stage.
 // FP64 storage: the reference's own operations.  FP32 storage (mixed form):
 // the kinetic energy, the energy/pressure subtraction and u are formed in FP64
 // from the stored values, then rounded for the FP32 reconstruction.
-template <int STAGE, typename S>
+template <int STAGE, typename S, bool G = false>
 __device__ __forceinline__ W6<S> prims(const StageArgs &a, const Raw<S> &v, unsigned int &flg)
 {
     W6<S> w;
-    if constexpr (sizeof(S) == 8) {
+    if constexpr (G && STAGE == 1) {
+        static_assert(sizeof(S) == 8, "gravity is FP64 only");
+        // source(0) (main.f90:289) with the cell's force: u_old = m/rho is the
+        // velocity of the last velocity call (the previous step's), the state
+        // at :314 takes it, the velocity at :318 the kicked momenta; seven
+        // quotients by rho with one shared reciprocal
+        const double r = v.r, E = v.E;
+        const double m[3] = {v.m0, v.m1, v.m2}, g[3] = {v.g0, v.g1, v.g2};
+        const double dth = a.dt * 0.5;
+        double mk[3];
+        grav_momenta(r, m, g, dth, mk);
+        const double num[7] = {m[0], m[1], m[2], mk[0], mk[1], mk[2],
+                               0.5 * (mk[0] * mk[0] + mk[1] * mk[1] + mk[2] * mk[2])};
+        double quo[7];
+        div_shared<7>(num, r, quo);
+        const double uo[3] = {quo[0], quo[1], quo[2]};
+        const double ek = quo[6];
+        const double E1 = floor_at(grav_energy_arg(r, E, ek, uo, g, dth), a.seps) + ek;
+        w.d = r;
+        w.u0 = quo[3];
+        w.u1 = quo[4];
+        w.u2 = quo[5];
+        // state (:314): pressure with u_old; the flux's eps with u (flux.f90)
+        w.p = floor_at(E1 - 0.5 * r * (uo[0] * uo[0] + uo[1] * uo[1] + uo[2] * uo[2]), a.seps) * a.gm1;
+        w.e = floor_at(E1 - 0.5 * r * (w.u0 * w.u0 + w.u1 * w.u1 + w.u2 * w.u2), a.seps);
+        if (fabs(w.u0) > a.vlimit || fabs(w.u1) > a.vlimit || fabs(w.u2) > a.vlimit) flg |= FLAG_CLAMP;
+    } else if constexpr (sizeof(S) == 8) {
         const double r = v.r, m0 = v.m0, m1 = v.m1, m2 = v.m2, E = v.E;
         w.d = r;
         // u = m/rho (velocity.f90:21) and, in stage 1, the kinetic energy of
@@ -278,10 +332,10 @@ __device__ __forceinline__ W6<S> prims(const StageArgs &a, const Raw<S> &v, unsi
     return w;
 }
 
-template <int STAGE, typename S>
+template <int STAGE, typename S, bool G = false>
 __device__ __forceinline__ W6<S> load_prims(const StageArgs &a, int64_t u, unsigned boff, unsigned int &flg)
 {
-    return prims<STAGE, S>(a, load_raw<STAGE, S>(a, u, boff), flg);
+    return prims<STAGE, S, G>(a, load_raw<STAGE, S, G>(a, u, boff), flg);
 }
 
 

@@ -135,7 +135,7 @@ struct TileGeom {
 // primitives of plane k are read back from the tile where needed, and the x
 // and z flux differences are parked in LDS until the update, to keep the
 // registers live across the barriers few.
-template <int STAGE, typename S, int TY>
+template <int STAGE, typename S, int TY, bool G>
 __device__ __forceinline__ void tile_owner(const StageArgs &a, TileLds<S, TY> &L, const TileGeom &g,
                                            unsigned int &flg, double &rmax)
 {
@@ -160,10 +160,10 @@ __device__ __forceinline__ void tile_owner(const StageArgs &a, TileLds<S, TY> &L
     Raw<S> nxt;
     const int kb = g.kb, ke = g.ke;
     if (!a.flatz) {
-        const W6<S> t0 = load_prims<STAGE, S>(a, zplane(kb - 2), coff, flg);
-        const W6<S> t1 = load_prims<STAGE, S>(a, zplane(kb - 1), coff, flg);
-        const W6<S> t2 = load_prims<STAGE, S>(a, zplane(kb), coff, flg);
-        const W6<S> t3 = load_prims<STAGE, S>(a, zplane(kb + 1), coff, flg);
+        const W6<S> t0 = load_prims<STAGE, S, G>(a, zplane(kb - 2), coff, flg);
+        const W6<S> t1 = load_prims<STAGE, S, G>(a, zplane(kb - 1), coff, flg);
+        const W6<S> t2 = load_prims<STAGE, S, G>(a, zplane(kb), coff, flg);
+        const W6<S> t3 = load_prims<STAGE, S, G>(a, zplane(kb + 1), coff, flg);
         Side<S> zmi, zpl;
         dir_states<2>(t0, t1, t2, sound_of(t1, a.gam), xy_ghost || zghost(kb - 1), a, zmi, zpl);
         zp_prev = zpl;
@@ -173,12 +173,12 @@ __device__ __forceinline__ void tile_owner(const StageArgs &a, TileLds<S, TY> &L
         zp_prev = zpl;
         store_w(L.P[kb & 1], c, t2);  // plane kb
         q2 = t3;
-        nxt = load_raw<STAGE, S>(a, zplane(kb + 2), coff);
+        nxt = load_raw<STAGE, S, G>(a, zplane(kb + 2), coff);
     } else {
-        const W6<S> w = load_prims<STAGE, S>(a, zplane(kb), coff, flg);
+        const W6<S> w = load_prims<STAGE, S, G>(a, zplane(kb), coff, flg);
         cs1 = sound_of(w, a.gam);
         store_w(L.P[kb & 1], c, w);
-        if (kb + 1 < ke) nxt = load_raw<STAGE, S>(a, zplane(kb + 1), coff);
+        if (kb + 1 < ke) nxt = load_raw<STAGE, S, G>(a, zplane(kb + 1), coff);
     }
     __syncthreads();  // every row's primitives of plane kb are in the tile
 
@@ -193,8 +193,8 @@ __device__ __forceinline__ void tile_owner(const StageArgs &a, TileLds<S, TY> &L
             const Raw<S> cur = nxt;
             // plane k+3 (the last iteration reloads a plane, so that the queue
             // needs no branch)
-            nxt = load_raw<STAGE, S>(a, zplane(min(k + 3, nz - 1)), coff);
-            q2 = prims<STAGE, S>(a, cur, flg);
+            nxt = load_raw<STAGE, S, G>(a, zplane(min(k + 3, nz - 1)), coff);
+            q2 = prims<STAGE, S, G>(a, cur, flg);
             cs1 = sound_of(q1, a.gam);
             Side<S> zmi1, zpl1;
             dir_states<2>(fetch_w(L.P[t], c), q1, q2, cs1, xy_ghost || zghost(k + 1), a, zmi1, zpl1);
@@ -210,8 +210,8 @@ __device__ __forceinline__ void tile_owner(const StageArgs &a, TileLds<S, TY> &L
             cs0 = cs1;
             if (k + 1 < ke) {
                 const Raw<S> cur = nxt;
-                nxt = load_raw<STAGE, S>(a, zplane(min(k + 2, ke - 1)), coff);
-                const W6<S> w = prims<STAGE, S>(a, cur, flg);
+                nxt = load_raw<STAGE, S, G>(a, zplane(min(k + 2, ke - 1)), coff);
+                const W6<S> w = prims<STAGE, S, G>(a, cur, flg);
                 cs1 = sound_of(w, a.gam);
                 store_w(L.P[t ^ 1], c, w);
             }
@@ -265,14 +265,14 @@ __device__ __forceinline__ void tile_owner(const StageArgs &a, TileLds<S, TY> &L
                 const double s = (a.ayz * L.Dx[i][o] + ey) + a.axy * dz;
                 ft[i] = -div_by(s * a.dt, a.vol, a.inv_vol, a.vol_pow2);
             }
-            stage_update<STAGE, S>(a, ui, uo, ft, fetch_w(L.P[t], c), cz, coff, flg, rmax);
+            stage_update<STAGE, S, G>(a, ui, uo, ft, fetch_w(L.P[t], c), cz, coff, flg, rmax);
         }
     }
 }
 
 // Halo rows -2, -1 and TY: their primitives, the y-states of rows -1 and TY,
 // and face y TY-1/2.
-template <int STAGE, typename S, int TY>
+template <int STAGE, typename S, int TY, bool G>
 __device__ __forceinline__ void tile_halo(const StageArgs &a, TileLds<S, TY> &L, const TileGeom &g,
                                           unsigned int &flg)
 {
@@ -288,16 +288,16 @@ __device__ __forceinline__ void tile_halo(const StageArgs &a, TileLds<S, TY> &L,
     const int c = (r + 2) * RS + lane + 1;
     const int kb = g.kb, ke = g.ke;
     Raw<S> nxt;
-    store_w(L.P[kb & 1], c, load_prims<STAGE, S>(a, zplane(kb), coff, flg));
-    if (kb + 1 < ke) nxt = load_raw<STAGE, S>(a, zplane(kb + 1), coff);
+    store_w(L.P[kb & 1], c, load_prims<STAGE, S, G>(a, zplane(kb), coff, flg));
+    if (kb + 1 < ke) nxt = load_raw<STAGE, S, G>(a, zplane(kb + 1), coff);
     __syncthreads();  // as the owners' first barrier
     for (int k = kb; k < ke; k++) {
         const int t = k & 1;
         if (k + 1 < ke) {
             // A: primitives of plane k+1 into the other tile
             const Raw<S> cur = nxt;
-            nxt = load_raw<STAGE, S>(a, zplane(min(k + 2, ke - 1)), coff);
-            store_w(L.P[t ^ 1], c, prims<STAGE, S>(a, cur, flg));
+            nxt = load_raw<STAGE, S, G>(a, zplane(min(k + 2, ke - 1)), coff);
+            store_w(L.P[t ^ 1], c, prims<STAGE, S, G>(a, cur, flg));
         }
         Side<S> ymi;
         if (r != -2) {
@@ -321,7 +321,7 @@ __device__ __forceinline__ void tile_halo(const StageArgs &a, TileLds<S, TY> &L,
 // Wave r=TY+1: its own halo row, and the x-edges of the owner rows one plane
 // ahead.  Lane 7*row+j holds column {-2,-1,0,62,63,64,65}[j] of owner row
 // `row`.
-template <int STAGE, typename S, int TY>
+template <int STAGE, typename S, int TY, bool G>
 __device__ __forceinline__ void tile_edge(const StageArgs &a, TileLds<S, TY> &L, const TileGeom &g,
                                           unsigned int &flg)
 {
@@ -347,7 +347,7 @@ __device__ __forceinline__ void tile_edge(const StageArgs &a, TileLds<S, TY> &L,
     W6<S> ew{};
     Side<S> emi{}, epl{};
     auto edge_prims = [&](int k, const Raw<S> &raw) {
-        ew = prims<STAGE, S>(a, raw, flg);
+        ew = prims<STAGE, S, G>(a, raw, flg);
         if (ej == 1 || ej == 5) store_w(L.P[k & 1], (erow + 2) * RS + ecol + 1, ew);
     };
     // neighbours within a row's seven lanes through whole-wave shifts (the
@@ -375,14 +375,14 @@ __device__ __forceinline__ void tile_edge(const StageArgs &a, TileLds<S, TY> &L,
 
     const int kb = g.kb, ke = g.ke;
     Raw<S> nxt;
-    store_w(L.P[kb & 1], c, load_prims<STAGE, S>(a, zplane(kb), coff, flg));
-    if (kb + 1 < ke) nxt = load_raw<STAGE, S>(a, zplane(kb + 1), coff);
+    store_w(L.P[kb & 1], c, load_prims<STAGE, S, G>(a, zplane(kb), coff, flg));
+    if (kb + 1 < ke) nxt = load_raw<STAGE, S, G>(a, zplane(kb + 1), coff);
     Raw<S> enxt{};
     if (e_on) {
-        edge_prims(kb, load_raw<STAGE, S>(a, zplane(kb), eoff));
+        edge_prims(kb, load_raw<STAGE, S, G>(a, zplane(kb), eoff));
         edge_states(kb);
         edge_publish();
-        if (kb + 1 < ke) enxt = load_raw<STAGE, S>(a, zplane(kb + 1), eoff);
+        if (kb + 1 < ke) enxt = load_raw<STAGE, S, G>(a, zplane(kb + 1), eoff);
     }
     __syncthreads();  // as the owners' first barrier
     for (int k = kb; k < ke; k++) {
@@ -390,11 +390,11 @@ __device__ __forceinline__ void tile_edge(const StageArgs &a, TileLds<S, TY> &L,
         if (more) {
             // A: primitives of plane k+1, its own row and the x-edges
             const Raw<S> cur = nxt;
-            nxt = load_raw<STAGE, S>(a, zplane(min(k + 2, ke - 1)), coff);
-            store_w(L.P[(k + 1) & 1], c, prims<STAGE, S>(a, cur, flg));
+            nxt = load_raw<STAGE, S, G>(a, zplane(min(k + 2, ke - 1)), coff);
+            store_w(L.P[(k + 1) & 1], c, prims<STAGE, S, G>(a, cur, flg));
             if (e_on) {
                 const Raw<S> ecur = enxt;
-                enxt = load_raw<STAGE, S>(a, zplane(min(k + 2, ke - 1)), eoff);
+                enxt = load_raw<STAGE, S, G>(a, zplane(min(k + 2, ke - 1)), eoff);
                 edge_prims(k + 1, ecur);
                 edge_states(k + 1);  // B
             }
@@ -406,7 +406,8 @@ __device__ __forceinline__ void tile_edge(const StageArgs &a, TileLds<S, TY> &L,
 }
 
 // One stage over the tile (bx, by) of 64 x TY outputs and the z planes of chunk bz.
-template <int STAGE, typename S, int TY>
+// G: with gravity (StageArgs::gf, FP64).
+template <int STAGE, typename S, int TY, bool G>
 __device__ __forceinline__ void tile_stage(const StageArgs &a, TileLds<S, TY> &L, unsigned bx, unsigned by,
                                            unsigned bz)
 {
@@ -431,10 +432,10 @@ __device__ __forceinline__ void tile_stage(const StageArgs &a, TileLds<S, TY> &L
         // win the issue arbitration by age: the owners issue first
         // (2.06 -> 2.00 ms per step at 256^3).
         __builtin_amdgcn_s_setprio(1);
-        tile_owner<STAGE, S, TY>(a, L, g, flg, rmax);
+        tile_owner<STAGE, S, TY, G>(a, L, g, flg, rmax);
     }
-    else if (g.r == TY + 1) tile_edge<STAGE, S, TY>(a, L, g, flg);
-    else tile_halo<STAGE, S, TY>(a, L, g, flg);
+    else if (g.r == TY + 1) tile_edge<STAGE, S, TY, G>(a, L, g, flg);
+    else tile_halo<STAGE, S, TY, G>(a, L, g, flg);
     if (flg) atomicOr(a.flags, flg);
     if (STAGE == 2) wave_max_to(rmax, a.rate);
 }

@@ -88,14 +88,43 @@ __device__ __forceinline__ void ydir_wave(const StageArgs &a, S *__restrict__ Ey
 // everything pointwise up to the next stage: stage 1 writes U* and the
 // stale-velocity pressure p* (q0: the cell's primitives of the stage input);
 // stage 2 writes U^{n+1} and accumulates the next step's Courant rate.
-template <int STAGE, typename S>
+//
+// G (gravity, FP64): source() applies the cell's force from a.gf, whose
+// planes the cell reads here again.  Stage 2 then also writes the pressure
+// the exact sequence leaves (its state call after source(1), main.f90:511-513,
+// takes the velocity from before the kick) to pout.
+template <int STAGE, typename S, bool G = false>
 __device__ __forceinline__ void stage_update(const StageArgs &a, const double *ui, const double *uo, const double *ft,
                                              const W6<S> &q0, int64_t c, unsigned coff, unsigned int &flg,
                                              double &rmax, bool park = false)
 {
     const int64_t n = a.n;
     double U[5];
-    if (STAGE == 1) {
+    double g[3] = {0.0, 0.0, 0.0};
+    const double dth = a.dt * 0.5;
+    if constexpr (G) {
+        static_assert(sizeof(S) == 8, "gravity is FP64 only");
+        for (int d = 0; d < 3; d++) g[d] = elem<double>(a.gf, (int64_t)d * n + c, coff);
+    }
+    if constexpr (G) {
+        // cons_old = source(0)(U^n) with the velocity of the last velocity
+        // call, u_old = m/rho (stage 1: cons = cons_old + fluxtmp, flux.f90:205;
+        // stage 2: cons = half*(cons_old+cons+fluxtmp), flux.f90:213)
+        const double *un = STAGE == 1 ? ui : uo;
+        const double r0 = un[0], m[3] = {un[1], un[2], un[3]};
+        double mk[3];
+        grav_momenta(r0, m, g, dth, mk);
+        const double num[4] = {m[0], m[1], m[2], 0.5 * (mk[0] * mk[0] + mk[1] * mk[1] + mk[2] * mk[2])};
+        double quo[4];
+        div_shared<4>(num, r0, quo);
+        const double ek = quo[3];
+        const double E1 = floor_at(grav_energy_arg(r0, un[4], ek, quo, g, dth), a.seps) + ek;
+        const double old[5] = {r0, mk[0], mk[1], mk[2], E1};
+        if (STAGE == 1)
+            for (int i = 0; i < 5; i++) U[i] = old[i] + ft[i];
+        else
+            for (int i = 0; i < 5; i++) U[i] = 0.5 * (old[i] + ui[i] + ft[i]);
+    } else if (STAGE == 1) {
         // cons = cons + fluxtmp (flux.f90:205) with cons = source(0)(U^n)
         const double r0 = ui[0], m0 = ui[1], m1 = ui[2], m2 = ui[3];
         const double ek = 0.5 * (m0 * m0 + m1 * m1 + m2 * m2) / r0;
@@ -140,16 +169,44 @@ __device__ __forceinline__ void stage_update(const StageArgs &a, const double *u
     } else {
         // main.f90:374-515: state, velocity, source(1), state, velocity
         if (U[0] < 1e-4) flg |= FLAG_LOWRHO;
-        // u = m/rho and the kinetic energy of the source re-round, quotients
-        // by rho with one shared reciprocal
-        const double num[4] = {U[1], U[2], U[3], 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3])};
-        double quo[4];
-        div_shared<4>(num, U[0], quo);
-        const double v0 = quo[0], v1 = quo[1], v2 = quo[2];
-        if (fabs(v0) > a.vlimit || fabs(v1) > a.vlimit || fabs(v2) > a.vlimit) flg |= FLAG_CLAMP;
-        const double ek = quo[3];
-        if (!(U[4] - ek > a.seps)) flg |= FLAG_FLOOR;
-        U[4] = floor_at(U[4] - ek, a.seps) + ek;
+        double v0, v1, v2;
+        if constexpr (G) {
+            // velocity u = m/rho, the kick of source(1), and the velocity of
+            // the kicked momenta: seven quotients by rho, one reciprocal
+            const double m[3] = {U[1], U[2], U[3]};
+            double mk[3];
+            grav_momenta(U[0], m, g, dth, mk);
+            const double num[7] = {m[0], m[1], m[2], mk[0], mk[1], mk[2],
+                                   0.5 * (mk[0] * mk[0] + mk[1] * mk[1] + mk[2] * mk[2])};
+            double quo[7];
+            div_shared<7>(num, U[0], quo);
+            if (fabs(quo[0]) > a.vlimit || fabs(quo[1]) > a.vlimit || fabs(quo[2]) > a.vlimit) flg |= FLAG_CLAMP;
+            const double ek = quo[6];
+            const double arg = grav_energy_arg(U[0], U[4], ek, quo, g, dth);
+            if (!(arg > a.seps)) flg |= FLAG_FLOOR;
+            U[4] = floor_at(arg, a.seps) + ek;
+            for (int d = 0; d < 3; d++) U[1 + d] = mk[d];
+            // the state after source(1) with the velocity from before it
+            const double ekp = U[0] * 0.5 * (quo[0] * quo[0] + quo[1] * quo[1] + quo[2] * quo[2]);
+            elem<S>(a.pout, c, coff) = floor_at(U[4] - ekp, a.seps) * a.gm1;
+            v0 = quo[3];
+            v1 = quo[4];
+            v2 = quo[5];
+            if (fabs(v0) > a.vlimit || fabs(v1) > a.vlimit || fabs(v2) > a.vlimit) flg |= FLAG_CLAMP;
+        } else {
+            // u = m/rho and the kinetic energy of the source re-round, quotients
+            // by rho with one shared reciprocal
+            const double num[4] = {U[1], U[2], U[3], 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3])};
+            double quo[4];
+            div_shared<4>(num, U[0], quo);
+            v0 = quo[0];
+            v1 = quo[1];
+            v2 = quo[2];
+            if (fabs(v0) > a.vlimit || fabs(v1) > a.vlimit || fabs(v2) > a.vlimit) flg |= FLAG_CLAMP;
+            const double ek = quo[3];
+            if (!(U[4] - ek > a.seps)) flg |= FLAG_FLOOR;
+            U[4] = floor_at(U[4] - ek, a.seps) + ek;
+        }
         S Us[5];
         for (int i = 0; i < 5; i++) {
             Us[i] = (S)U[i];
@@ -339,7 +396,7 @@ __device__ __forceinline__ void materialize_cell(int64_t n, int64_t c, const dou
     u[n + c] = v1;
     u[2 * n + c] = v2;
     const double ekin = r * 0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
-    p[c] = vmax(cons[4 * n + c] - ekin, seps) * (gam - 1.0);
+    if (p) p[c] = vmax(cons[4 * n + c] - ekin, seps) * (gam - 1.0);
     ad[c] = gam;
 }
 

@@ -67,6 +67,16 @@ def main() -> None:
                                         10, 1, 0, 2)
     walls = HydroParams(nx=14, ny=16, nz=10, dx=0.2, dy=0.2, dz=0.2, endtime=1e30, cfl=0.4, gammafix=1.4)
     cases["grav_ext_walls3d"] = record(walls, {"kind": "gravity", "seed": 4, "kw": {"omega": 0.3}}, 8, 1, 0, 2)
+    # a potential alone at sizes of several tiles and z-chunks (the fused
+    # step's gravity kernels): power-of-two spacings, x walls, a walled flat grid
+    big = HydroParams(nx=72, ny=44, nz=38, dx=0.0625, dy=0.0625, dz=0.0625, endtime=1e30, cfl=0.4,
+                      gammafix=5.0 / 3.0, **OUTFLOW)
+    cases["grav_pot_box72"] = record(big, {"kind": "gravity", "seed": 6, "kw": {"omega": 0.5}}, 12, 4, 0, 1)
+    xw = dict(OUTFLOW, no_outflow_xl=True, no_outflow_xr=True)
+    xwalls = HydroParams(nx=66, ny=40, nz=30, dx=0.1, dy=0.1, dz=0.1, endtime=1e30, cfl=0.4, gammafix=1.4, **xw)
+    cases["grav_pot_xwalls"] = record(xwalls, {"kind": "gravity", "seed": 7, "kw": {"omega": 0.4}}, 10, 4, 0, 1)
+    flat = HydroParams(nx=100, ny=70, nz=4, dx=0.08, dy=0.08, dz=0.08, endtime=1e30, cfl=0.4, gammafix=1.4)
+    cases["grav_pot_flat_walls"] = record(flat, {"kind": "gravity", "seed": 8, "kw": {"omega": 0.6}}, 10, 4, 0, 1)
     with open(OUT, "w") as f:
         json.dump(cases, f, indent=1)
     for k, r in cases.items():

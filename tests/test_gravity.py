@@ -41,22 +41,62 @@ def test_generator_reproduces_golden_inputs(name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["exact", "fused"])
+@pytest.mark.parametrize("mode", ["exact", "fused", "graph"])
 @pytest.mark.parametrize("name", sorted(GRAV))
 def test_gpu_steps_match_reference(name, mode):
     """Sequential mass moments: bit-identical to the reference on one thread.
-    The fused mode runs the exact sequence when gravity is configured."""
+    With a potential alone the fused and graph modes take the fused kernels
+    after the first step (from the set-up); with the point mass, whose force
+    needs the corrector's mass moments, they run the exact sequence."""
     from boxzyhydro_amd import native as nv
 
     rec = GRAV[name]
     print(f"gravity case {name} {mode}", flush=True)
     _, g = _grid(rec)
-    dts = g.run(rec["steps"], mode=nv.BX_STEP_EXACT if mode == "exact" else nv.BX_STEP_FUSED)
+    m = {"exact": nv.BX_STEP_EXACT, "fused": nv.BX_STEP_FUSED, "graph": nv.BX_STEP_GRAPH}[mode]
+    dts = g.run(rec["steps"], mode=m)
     assert [float(x) for x in dts] == rec["dt"]
     out = g.download()
     for k in ("cons", "u", "p"):
         assert sha(out[k]) == rec["final_sha256"][k], k
+    fused, exact = g.step_counts()
+    print(f"  fused steps {fused}, exact steps {exact}, fallbacks {g.fallback_count()}", flush=True)
+    if mode == "exact" or rec["grav"] == 2:
+        assert fused == 0
+    else:
+        assert exact == 1 and g.fallback_count() == 0 and fused == rec["steps"] - 1
+        if mode == "graph" and rec["steps"] >= 5:
+            assert g.graph_step_count() > 0
     g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fused", "graph"])
+def test_gpu_fused_potential_change_mid_run(mode):
+    """A new phi between calls: the next step's source(0) still takes the
+    force of the old phi and its calc_gforce that of the new one, so that step
+    runs the exact sequence; the fused kernels resume after it.  Bit-identical
+    to the exact mode making the same calls."""
+    from boxzyhydro_amd import native as nv
+
+    rec = GRAV["grav_pot_xwalls"]
+    prm, ic = case_inputs(rec)
+    phi2 = 0.7 * ic["phi"] + 0.05 * np.cos(np.arange(ic["phi"].size) * 0.01)
+    outs = {}
+    for m in ("exact", mode):
+        print(f"gravity phi change {m}", flush=True)
+        _, g = _grid(rec)
+        step = nv.BX_STEP_EXACT if m == "exact" else (nv.BX_STEP_FUSED if m == "fused" else nv.BX_STEP_GRAPH)
+        d1 = g.run(4, mode=step)
+        nv.check(nv.lib().bx_set_phi(g._h, nv.dptr(np.ascontiguousarray(phi2))), "bx_set_phi")
+        d2 = g.run(6, mode=step)
+        out = g.download()
+        outs[m] = (list(d1) + list(d2), {k: sha(out[k]) for k in ("cons", "u", "p")})
+        if m != "exact":
+            fused, exact = g.step_counts()
+            assert exact == 2 and fused == 8 and g.fallback_count() == 0, (fused, exact)
+        g.close()
+    assert outs["exact"] == outs[mode]
 
 
 @pytest.mark.gpu
