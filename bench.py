@@ -534,16 +534,22 @@ def options_cost(n: int, steps: int):
     x, y, z = ics.cell_centres(prm)
     phi = (-1.0 / np.sqrt(x[None, None, :] ** 2 + y[None, :, None] ** 2 + z[:, None, None] ** 2 + 100.0)).reshape(-1)
 
-    def step_ms(p, grav=0):
+    def step_ms(p, grav=0, mode=nv.BX_STEP_EXACT, k=steps):
         g = HydroGrid(p, ic, device=0)
         if grav:
             g.set_gravity(grav, phi, object_mass=1.0, object_radius=20.0)
-        g.run(1, mode=nv.BX_STEP_EXACT)
+        g.run(1 if mode == nv.BX_STEP_EXACT else 4, mode=mode)
         g.sync()
+        f0, _ = g.step_counts()
         t0 = time.perf_counter()
-        g.run(steps, mode=nv.BX_STEP_EXACT)
+        g.run(k, mode=mode)
         g.sync()
-        ms = (time.perf_counter() - t0) / steps * 1e3
+        ms = (time.perf_counter() - t0) / k * 1e3
+        if mode != nv.BX_STEP_EXACT:  # every timed step took the fused kernels
+            f1, _ = g.step_counts()
+            assert f1 - f0 == k and g.fallback_count() == 0, (f1 - f0, k)
+            g.close()
+            return ms, None
         moments = None
         if grav & nv.BX_GRAV_EXTERNAL:
             g.external_grav()
@@ -558,10 +564,22 @@ def options_cost(n: int, steps: int):
     ang, _ = step_ms(dataclasses.replace(prm, fluxangmom=True))
     tree, tree_call = step_ms(prm, nv.BX_GRAV_POTENTIAL | nv.BX_GRAV_EXTERNAL)
     seq, seq_call = step_ms(prm, nv.BX_GRAV_POTENTIAL | nv.BX_GRAV_EXTERNAL | nv.BX_GRAV_COM_SEQUENTIAL)
+    # a potential alone: the fused kernels (bit-identical to the exact sequence)
+    pot, _ = step_ms(prm, nv.BX_GRAV_POTENTIAL)
+    kf = 4 * steps
+    pot_fused, _ = step_ms(prm, nv.BX_GRAV_POTENTIAL, nv.BX_STEP_FUSED, kf)
+    pot_graph, _ = step_ms(prm, nv.BX_GRAV_POTENTIAL, nv.BX_STEP_GRAPH, kf)
+    plain_graph, _ = step_ms(prm, 0, nv.BX_STEP_GRAPH, kf)
     return {"workload": f"rotating box {n}^3 interior FP64, exact call sequence, {steps} steps each",
             "exact_step_ms": base, "flux_ang_step_ms": ang, "flux_ang_over_plain": ang / base,
             "gravity_tree_step_ms": tree, "gravity_sequential_step_ms": seq,
             "external_grav_call_ms": {"tree": tree_call, "sequential": seq_call},
+            "potential": {"exact_step_ms": pot, "fused_step_ms": pot_fused, "graph_step_ms": pot_graph,
+                          "plain_graph_step_ms": plain_graph, "graph_over_plain_graph": pot_graph / plain_graph,
+                          "exact_over_graph": pot / pot_graph,
+                          "note": f"BX_GRAV_POTENTIAL alone, {kf} fused steps after 4 untimed; fused = one step "
+                                  "per host round trip, graph = captured step pairs; plain = the same box "
+                                  "without gravity"},
             "note": "gravity = calc_gforce + external_grav before source(tphase,1) each step; the sequential "
                     "mass moments (bit-identical to a one-thread reference) run on one wave"}
 

@@ -65,6 +65,7 @@ extern "C" int bx_step_static(bx_ctx *c, const double *cons_in, double *cons_out
                                          std::to_string((long long)BX_STATIC_MAX_CELLS) + " cells");
     if (!(dt > 0.0) || !std::isfinite(dt)) return set_error(BX_ERR_ARG, "bx_step_static: dt must be positive and finite");
     if (int rc = fused_supported(c)) return rc;
+    if (c->grav_flags) return set_error(BX_ERR_UNSUPPORTED, "bx_step_static: no gravity configuration (zero potential)");
     BX_HIP(hipSetDevice(c->device));
     unsigned char *base = nullptr;
     if (int rc = scratch_ptr(c->device, &base)) return rc;

@@ -415,9 +415,16 @@ int bx_download_muc(bx_ctx *ctx, double *muc_array);
  * applies gforce (source.f90:148-182).  phi itself comes from its
  * self-gravity solver, which is not part of this library: here it is given.
  * With flags set, bx_step makes the main-loop calls (main.f90:479-487:
- * calc_gforce, then external_grav, before source(tphase,1)) and runs the
- * exact sequence in either mode.  The caller makes the calls before the
- * loop (main.f90:146-154) with bx_calc_gforce / bx_external_grav.
+ * calc_gforce, then external_grav, before source(tphase,1)).  The caller
+ * makes the calls before the loop (main.f90:146-154) with bx_calc_gforce /
+ * bx_external_grav.  With BX_GRAV_POTENTIAL alone (and no two-level corner
+ * ghosts), the fused and graph modes take the fused kernels, bit-identical
+ * to the exact sequence, whenever gforce is the force of the current phi;
+ * otherwise (after bx_set_phi, before the next calc_gforce) that step runs
+ * the exact sequence, whose calc_gforce settles it.  With BX_GRAV_EXTERNAL
+ * every step runs the exact sequence (the point mass's force needs the
+ * corrector's mass moments before source(tphase,1)).  The allocation-free
+ * step (bx_step_static) takes no gravity configuration.
  */
 #define BX_GRAV_POTENTIAL 1 /* calc_gforce from phi (USEPOTENTIAL) */
 #define BX_GRAV_EXTERNAL 2  /* external_grav point mass (EXTERNALGRAV) */
