@@ -268,8 +268,6 @@ int fused_supported(const bx_ctx *c)
     if (c->grav_flags & BX_GRAV_EXTERNAL)
         return set_error(BX_ERR_UNSUPPORTED, "fused step with gravity: a given potential only (no point mass, whose "
                                              "force needs the corrector's mass moments before source(1))");
-    if (c->grav_flags && c->nd2 > 0)
-        return set_error(BX_ERR_UNSUPPORTED, "fused step with gravity: no two-level corner ghosts");
     if (c->eos) return set_error(BX_ERR_UNSUPPORTED, "fused step assumes a fixed gamma (H2STAT=-1)");
     return BX_OK;
 }
@@ -424,21 +422,29 @@ __global__ __launch_bounds__(64) void k_corner_ghosts(StageArgs a0, const int32_
         }
         double rv = 0.0;
         // rewrites U^{n+1} of v with the bits the tiled kernel wrote and parks
-        // the result before source(1) in d2ubar
-        stage_update<2, double>(a, ui, uo, ft, q0, v, 0u, flg, rv, true);
+        // the result before source(1) in d2ubar (with gravity also p of v)
+        if (a.gf) stage_update<2, double, true>(a, ui, uo, ft, q0, v, 0u, flg, rv, true);
+        else stage_update<2, double>(a, ui, uo, ft, q0, v, 0u, flg, rv, true);
         double U[5];
         for (int i = 0; i < 5; i++) U[i] = a.d2ubar[6 * j + i];
         // the pressure the exact sequence leaves in the ghost: the state call
         // after the last ghost pair (main.f90:511-513) takes the velocity of
         // the previous velocity call, when the ghost held h2's value from
         // before the third pair, 0.5*(cons_old + U* + 0) of v up to signs
-        // (flux.f90:213, cons_old = U^n after source(0), which leaves rho and
-        // m as they are); the signs drop out of |u|^2
+        // (flux.f90:213, cons_old = U^n after source(0), which leaves rho as
+        // it is and, without gravity, m too; with gravity m is kicked by v's
+        // force); the signs drop out of |u|^2
         {
             double rc = 0.5 * ((uo[0] + ui[0]) + 0.0);
             rc = vmax(rc, a.srho);
+            double mo[3] = {uo[1], uo[2], uo[3]};
+            if (a.gf) {
+                const double g[3] = {a.gf[v], a.gf[a.n + v], a.gf[2 * a.n + v]};
+                const double m[3] = {uo[1], uo[2], uo[3]};
+                grav_momenta(uo[0], m, g, a.dt * 0.5, mo);
+            }
             double uc[3];
-            for (int d = 0; d < 3; d++) uc[d] = (0.5 * ((uo[1 + d] + ui[1 + d]) + 0.0)) / rc;
+            for (int d = 0; d < 3; d++) uc[d] = (0.5 * ((mo[d] + ui[1 + d]) + 0.0)) / rc;
             const double ek = U[0] * 0.5 * (uc[0] * uc[0] + uc[1] * uc[1] + uc[2] * uc[2]);
             a.d2ubar[6 * j + 5] = vmax(U[4] - ek, a.seps) * a.gm1;
         }

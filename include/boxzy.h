@@ -417,8 +417,8 @@ int bx_download_muc(bx_ctx *ctx, double *muc_array);
  * With flags set, bx_step makes the main-loop calls (main.f90:479-487:
  * calc_gforce, then external_grav, before source(tphase,1)).  The caller
  * makes the calls before the loop (main.f90:146-154) with bx_calc_gforce /
- * bx_external_grav.  With BX_GRAV_POTENTIAL alone (and no two-level corner
- * ghosts), the fused and graph modes take the fused kernels, bit-identical
+ * bx_external_grav.  With BX_GRAV_POTENTIAL alone, the fused and graph
+ * modes take the fused kernels (walls on any sides), bit-identical
  * to the exact sequence, whenever gforce is the force of the current phi;
  * otherwise (after bx_set_phi, before the next calc_gforce) that step runs
  * the exact sequence, whose calc_gforce settles it.  With BX_GRAV_EXTERNAL

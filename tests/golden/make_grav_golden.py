@@ -77,6 +77,11 @@ def main() -> None:
     cases["grav_pot_xwalls"] = record(xwalls, {"kind": "gravity", "seed": 7, "kw": {"omega": 0.4}}, 10, 4, 0, 1)
     flat = HydroParams(nx=100, ny=70, nz=4, dx=0.08, dy=0.08, dz=0.08, endtime=1e30, cfl=0.4, gammafix=1.4)
     cases["grav_pot_flat_walls"] = record(flat, {"kind": "gravity", "seed": 8, "kw": {"omega": 0.6}}, 10, 4, 0, 1)
+    # walls on every side (the shipped in.nml): 3-D, with two-level corner ghosts.  One
+    # thread: ghosts whose donors are ghosts take values that depend on the order of the
+    # reference's OpenMP ghost fill
+    allw = HydroParams(nx=40, ny=36, nz=30, dx=0.1, dy=0.1, dz=0.1, endtime=1e30, cfl=0.4, gammafix=1.4)
+    cases["grav_pot_walls3d"] = record(allw, {"kind": "gravity", "seed": 9, "kw": {"omega": 0.5}}, 10, 1, 0, 1)
     with open(OUT, "w") as f:
         json.dump(cases, f, indent=1)
     for k, r in cases.items():
