@@ -1006,6 +1006,9 @@ extern "C" int bx_tune(bx_ctx *c, int32_t y_chunk, int32_t z_chunk, int32_t occu
     if (y_chunk < 1 || z_chunk < 1) return set_error(BX_ERR_ARG, "bx_tune: chunks must be >= 1");
     if (occupancy != 0 && occupancy != 1 && occupancy != 4 && !(occupancy >= 10 && occupancy <= 11))
         return set_error(BX_ERR_ARG, "bx_tune: variant 0, 1, 4, 10 or 11");
+    if (c->grav_flags && occupancy != 11)
+        return set_error(BX_ERR_UNSUPPORTED, "bx_tune: with gravity the fused step has the 64 x 8 tiled kernel only "
+                                             "(variant 11)");
     c->fused_ly = y_chunk;
     c->fused_lz = z_chunk;
     c->fused_occ = occupancy;

@@ -267,6 +267,12 @@ extern "C" int bx_gravity_config(bx_ctx *c, int32_t flags, double object_mass, d
     c->obj_radius = object_radius;
     c->phi_gen++;  // gforce (zero) is not yet the force of phi
     c->fused_valid = false;
+    if (c->fused_lz != 0 && c->fused_occ != 11) {
+        // a tuned variant other than the 64 x 8 tiled kernel, which alone
+        // has the gravity form: back to the default work split
+        c->fused_ly = c->fused_lz = 0;
+        c->fused_occ = 0;
+    }
     // fused_prepare decides on the next fused step: a potential alone takes
     // the fused kernels, the point mass the exact sequence
     c->fused_unsupported = false;

@@ -155,12 +155,14 @@ int bx_graph_step_count(const bx_ctx *ctx, int64_t *count);
 int bx_step_counts(const bx_ctx *ctx, int64_t *fused, int64_t *exact);
 
 /* Work split of the fused stage kernels.  variant 11 (the FP64 default) and
- * 10: one kernel per stage over x-y tiles of 64 x 8 (11) or 64 x 4 (10)
- * cells marching z_chunk planes in z (y_chunk unused).  Variants 0, 1, 4
- * (1 is the mixed form's default): two kernels per stage, a y-direction
- * kernel of y_chunk rows per wave and an x/z kernel of z_chunk planes per
- * wave (1: y-term loaded one plane early; 4: built for four waves per SIMD).
- * Used by the autotuner; results do not change. */
+ * 10 (the mixed form's default): one kernel per stage over x-y tiles of
+ * 64 x 8 (11) or 64 x 4 (10) cells marching z_chunk planes in z (y_chunk
+ * unused).  Variants 0, 1, 4: two kernels per stage, a y-direction kernel of
+ * y_chunk rows per wave and an x/z kernel of z_chunk planes per wave (1:
+ * y-term loaded one plane early; 4: built for four waves per SIMD).  Used by
+ * the autotuner; results do not change.  With gravity configured only
+ * variant 11 is accepted (BX_ERR_UNSUPPORTED otherwise), and configuring
+ * gravity returns another tuned variant to the default. */
 int bx_tune(bx_ctx *ctx, int32_t y_chunk, int32_t z_chunk, int32_t variant);
 
 /* Event timing of the fused stage kernels on the context's stream.

@@ -186,3 +186,31 @@ def test_gpu_external_point_mass_far_field():
     np.testing.assert_allclose(gf[far, 0], 2 * (-mass * X[far] / r[far] ** 3), rtol=1e-13)
     np.testing.assert_allclose(gf[far, 1], 2 * (-mass * Y[far] / r[far] ** 3), rtol=1e-13)
     assert np.all(gf[:, 2] == 0.0)
+
+
+@pytest.mark.gpu
+def test_gpu_tuned_variant_with_gravity():
+    """Only the tiled 64 x 8 stage kernel has the gravity form: configuring a
+    potential returns an earlier tuned variant to the default split, bx_tune
+    then refuses the others, and the run stays bit-identical to the
+    reference."""
+    from boxzyhydro_amd import native as nv
+    from boxzyhydro_amd.hydro import HydroGrid
+
+    rec = GRAV["grav_pot_xwalls"]
+    prm, ic = case_inputs(rec)
+    phi = ic.pop("phi")
+    print("gravity with a tuned variant", flush=True)
+    g = HydroGrid(prm, ic, device=0)
+    g.tune(8, 16, 1)  # a two-kernel variant
+    g.set_gravity(nv.BX_GRAV_POTENTIAL, phi)
+    with pytest.raises(nv.BoxzyError, match="variant 11"):
+        g.tune(8, 16, 4)
+    g.tune(1, 8, 11)
+    dts = g.run(rec["steps"], mode=nv.BX_STEP_FUSED)
+    assert [float(x) for x in dts] == rec["dt"]
+    out = g.download()
+    for k in ("cons", "u", "p"):
+        assert sha(out[k]) == rec["final_sha256"][k], k
+    assert g.step_counts() == (rec["steps"] - 1, 1) and g.fallback_count() == 0
+    g.close()
